@@ -1,0 +1,123 @@
+/*
+ * p3dfft.h -- C interface of libp3dfft_amd (MI355X / gfx950).
+ *
+ * Drop-in for the reference's C header (include/p3dfft.h of P3DFFT 2.7.x) on
+ * the ftran_r2c / btran_c2r path: the same entry-point names, argument order
+ * and by-reference conventions, and the same inline Cp3dfft_* convenience
+ * wrappers.  Link libp3dfft_amd.so (double) or, with SINGLE_PREC defined,
+ * libp3dfft_amd_sp.so (float).
+ *
+ * Each declaration cites the reference entry point it replaces (paths in the
+ * reference tree).
+ */
+#ifndef P3DFFT_AMD_H
+#define P3DFFT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifdef SINGLE_PREC
+typedef float p3dfft_real;
+#else
+typedef double p3dfft_real;
+#endif
+
+/* ---- the reference's BIND(C) entry points ---------------------------- */
+
+/* build/setup.F90:83 (p3dfft_setup_c, BIND(C,NAME='p3dfft_setup')).
+ * dims: processor grid, must be {1,1} (one GPU = one rank); comm: ignored;
+ * nxc/nyc/nzc: cut (pruned) sizes, 1..n; ow: nonzero allows btran to
+ * overwrite its input; memsize: 3 ints out. */
+void p3dfft_setup(int *dims, int *nx, int *ny, int *nz, int *comm, int *nxc, int *nyc, int *nzc, int *ow,
+                  int *memsize);
+
+/* build/module.F90:214 (p3dfft_get_dims_w).  conf 1: real X-pencil, 2: complex
+ * Z-pencil (order (Z,Y,X) under the stride-1 layout), 3: padded in-place size. */
+void p3dfft_get_dims(int *start, int *end, int *size, int *conf);
+
+/* build/ftran.F90:469 (p3dfft_ftran_r2c_w).  A: real (nx,ny,nz) host array;
+ * B: complex (nxhpc,nyc,nzc) host array as interleaved reals; op: 3 chars, the
+ * third selects the Z transform: 't'/'f' FFT, 'c' DCT-I, 's' DST-I, 'n'/'0' none. */
+void p3dfft_ftran_r2c(p3dfft_real *A, p3dfft_real *B, unsigned char *op);
+
+/* build/btran.F90:377 (p3dfft_btran_c2r_w).  op(1:1) selects the Z transform. */
+void p3dfft_btran_c2r(p3dfft_real *A, p3dfft_real *B, unsigned char *op);
+
+/* build/ftran.F90:359 (p3dfft_cheby_w): 'ffc' forward, scaling and the
+ * Chebyshev derivative recurrence along Z. */
+void p3dfft_cheby(p3dfft_real *A, p3dfft_real *B, p3dfft_real *Lz);
+
+/* build/module.F90:301 (p3dfft_clean_w) */
+void p3dfft_clean(void);
+
+/* build/module.F90:709 / :719 (get_timers_w / set_timers_w): 12 accumulating
+ * stage timers in seconds, filled from HIP events. */
+void get_timers(double *timers);
+void set_timers(void);
+
+/* ---- inline C convenience layer (reference include/p3dfft.h:165-264) --- */
+static inline void Cp3dfft_setup(int *dims, int nx, int ny, int nz, int comm, int nxc, int nyc, int nzc,
+                                 int overwrite, int *memsize) {
+  p3dfft_setup(dims, &nx, &ny, &nz, &comm, &nxc, &nyc, &nzc, &overwrite, memsize);
+}
+static inline void Cp3dfft_clean(void) { p3dfft_clean(); }
+static inline void Cp3dfft_get_dims(int *start, int *end, int *size, int conf) {
+  p3dfft_get_dims(start, end, size, &conf);
+}
+static inline void Cget_timers(double *timers) { get_timers(timers); }
+static inline void Cset_timers(void) { set_timers(); }
+static inline void Cp3dfft_ftran_r2c(p3dfft_real *A, p3dfft_real *B, unsigned char *op) {
+  p3dfft_ftran_r2c(A, B, op);
+}
+static inline void Cp3dfft_btran_c2r(p3dfft_real *A, p3dfft_real *B, unsigned char *op) {
+  p3dfft_btran_c2r(A, B, op);
+}
+static inline void Cp3dfft_cheby(p3dfft_real *A, p3dfft_real *B, p3dfft_real Lz) { p3dfft_cheby(A, B, &Lz); }
+
+/* ---- extensions (no counterpart in the reference) -------------------- */
+
+/* Device-pointer variants of ftran/btran/cheby: in/out are HBM addresses laid
+ * out as the host arrays above; the work is queued on `stream` (a hipStream_t,
+ * NULL = default stream) and the call returns without synchronising.
+ * Return 0, or -1 on error (message via p3dfft_amd_last_error). */
+int p3dfft_ftran_r2c_dev(const p3dfft_real *in, p3dfft_real *out, const unsigned char *op, void *stream);
+int p3dfft_btran_c2r_dev(p3dfft_real *in, p3dfft_real *out, const unsigned char *op, void *stream);
+int p3dfft_cheby_dev(const p3dfft_real *in, p3dfft_real *out, p3dfft_real Lz, void *stream);
+
+/* Select the (Z,Y,X) complex layout for the next p3dfft_setup: the
+ * reference's compile-time STRIDE1 option as a run-time switch. */
+void p3dfft_amd_set_stride1(int on);
+
+/* Error handling: mode 0 (default) prints and terminates the process on the
+ * errors where the reference calls MPI_Abort; mode 1 prints, records the
+ * message and returns. */
+void p3dfft_amd_set_error_mode(int mode);
+const char *p3dfft_amd_last_error(void);
+void p3dfft_amd_clear_error(void);
+
+/* Stage timers on (default) or off. */
+void p3dfft_amd_set_timing(int on);
+/* sizeof(p3dfft_real) of this build: 8 or 4. */
+int p3dfft_amd_precision(void);
+
+/* HBM helpers for callers without their own HIP bindings. */
+void *p3dfft_amd_malloc(size_t bytes);
+int p3dfft_amd_free(void *p);
+int p3dfft_amd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int p3dfft_amd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int p3dfft_amd_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int p3dfft_amd_memset(void *dst, int value, size_t bytes);
+int p3dfft_amd_synchronize(void);
+int p3dfft_amd_device_count(void);
+void *p3dfft_amd_event_create(void);
+int p3dfft_amd_event_record(void *event, void *stream);
+float p3dfft_amd_event_elapsed_ms(void *start, void *stop);
+int p3dfft_amd_event_destroy(void *event);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

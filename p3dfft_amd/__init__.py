@@ -1,0 +1,257 @@
+"""p3dfft_amd -- MI355X (gfx950) implementation of P3DFFT's forward/backward
+3D FFT path behind P3DFFT's own C-ABI.
+
+The product is the shared library built from p3dfft_amd/csrc:
+  lib/libp3dfft_amd.so     double precision (reference default build)
+  lib/libp3dfft_amd_sp.so  single precision (reference SINGLE_PREC build)
+Both export the reference's BIND(C) entry points (p3dfft_setup,
+p3dfft_get_dims, p3dfft_ftran_r2c, p3dfft_btran_c2r, p3dfft_cheby,
+p3dfft_clean, get_timers, set_timers) plus device-pointer extensions.
+
+This module is a thin ctypes mirror of that interface for Python callers
+(tests, benchmark, serving loop).  It never falls back to a CPU path: if the
+library is missing it raises.
+
+Layout conventions (Fortran order of the reference, first index fastest): a
+real field (nx, ny, nz) is a numpy array of shape (nz, ny, nx); the complex
+spectrum (nxhpc, nyc, nzc) has shape (nzc, nyc, nxhpc); under STRIDE1 the
+spectrum (nzc, nyc, nxhpc) has shape (nxhpc, nyc, nzc).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(_HERE, "lib")
+_LIBNAMES = {8: "libp3dfft_amd.so", 4: "libp3dfft_amd_sp.so"}
+_libs = {}
+
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+class P3DFFTError(RuntimeError):
+    pass
+
+
+def lib_path(precision=8):
+    return os.path.join(LIBDIR, _LIBNAMES[precision])
+
+
+def load(precision=8):
+    """Load (once) the library of the given precision (8 = double, 4 = single)."""
+    if precision in _libs:
+        return _libs[precision]
+    path = lib_path(precision)
+    if not os.path.exists(path):
+        raise P3DFFTError("p3dfft_amd library not built: %s (run __graft_entry__.build())" % path)
+    L = ctypes.CDLL(path)
+    rp = ctypes.POINTER(ctypes.c_double if precision == 8 else ctypes.c_float)
+    rt = ctypes.c_double if precision == 8 else ctypes.c_float
+    L.p3dfft_setup.argtypes = [_ip] * 10
+    L.p3dfft_setup.restype = None
+    L.p3dfft_get_dims.argtypes = [_ip] * 4
+    for n in ("p3dfft_ftran_r2c", "p3dfft_btran_c2r"):
+        getattr(L, n).argtypes = [_vp, _vp, ctypes.c_char_p]
+        getattr(L, n).restype = None
+    L.p3dfft_cheby.argtypes = [_vp, _vp, rp]
+    L.p3dfft_clean.argtypes = []
+    L.get_timers.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    L.set_timers.argtypes = []
+    for n in ("p3dfft_ftran_r2c_dev", "p3dfft_btran_c2r_dev"):
+        getattr(L, n).argtypes = [_vp, _vp, ctypes.c_char_p, _vp]
+        getattr(L, n).restype = ctypes.c_int
+    L.p3dfft_cheby_dev.argtypes = [_vp, _vp, rt, _vp]
+    L.p3dfft_cheby_dev.restype = ctypes.c_int
+    L.p3dfft_amd_set_stride1.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_set_error_mode.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_last_error.restype = ctypes.c_char_p
+    L.p3dfft_amd_set_timing.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_precision.restype = ctypes.c_int
+    L.p3dfft_amd_malloc.argtypes = [ctypes.c_size_t]
+    L.p3dfft_amd_malloc.restype = _vp
+    L.p3dfft_amd_free.argtypes = [_vp]
+    for n in ("p3dfft_amd_memcpy_h2d", "p3dfft_amd_memcpy_d2h", "p3dfft_amd_memcpy_d2d"):
+        getattr(L, n).argtypes = [_vp, _vp, ctypes.c_size_t]
+        getattr(L, n).restype = ctypes.c_int
+    L.p3dfft_amd_memset.argtypes = [_vp, ctypes.c_int, ctypes.c_size_t]
+    L.p3dfft_amd_synchronize.restype = ctypes.c_int
+    L.p3dfft_amd_device_count.restype = ctypes.c_int
+    L.p3dfft_amd_event_create.restype = _vp
+    L.p3dfft_amd_event_record.argtypes = [_vp, _vp]
+    L.p3dfft_amd_event_elapsed_ms.argtypes = [_vp, _vp]
+    L.p3dfft_amd_event_elapsed_ms.restype = ctypes.c_float
+    L.p3dfft_amd_event_destroy.argtypes = [_vp]
+    if L.p3dfft_amd_precision() != precision:
+        raise P3DFFTError("library %s reports precision %d" % (path, L.p3dfft_amd_precision()))
+    _libs[precision] = L
+    return L
+
+
+class DeviceBuffer:
+    """HBM allocation owned by the library's HIP runtime."""
+
+    def __init__(self, lib, nbytes):
+        self.lib = lib
+        self.nbytes = int(nbytes)
+        self.ptr = lib.p3dfft_amd_malloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise P3DFFTError("device allocation of %d bytes failed" % self.nbytes)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise P3DFFTError("upload larger than buffer")
+        if self.lib.p3dfft_amd_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes):
+            raise P3DFFTError("H2D copy failed")
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise P3DFFTError("download larger than buffer")
+        if self.lib.p3dfft_amd_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes):
+            raise P3DFFTError("D2H copy failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.lib.p3dfft_amd_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class P3DFFT:
+    """Mirror of the reference's API for one setup (the library keeps one
+    active setup, as the reference does)."""
+
+    def __init__(self, precision=8):
+        self.precision = precision
+        self.lib = load(precision)
+        self.rdtype = np.float64 if precision == 8 else np.float32
+        self.cdtype = np.complex128 if precision == 8 else np.complex64
+        self.active = False
+
+    # -- reference API ----------------------------------------------------
+    def setup(self, dims, nx, ny, nz, nxc=None, nyc=None, nzc=None, overwrite=True, stride1=False,
+              strict=True):
+        """p3dfft_setup (build/setup.F90:83-104).  Returns memsize."""
+        L = self.lib
+        L.p3dfft_amd_set_error_mode(1 if strict else 0)
+        L.p3dfft_amd_clear_error()
+        L.p3dfft_amd_set_stride1(1 if stride1 else 0)
+        d = (ctypes.c_int * 2)(*dims)
+        ms = (ctypes.c_int * 3)()
+        args = [nx, ny, nz, 0, nxc or nx, nyc or ny, nzc or nz, 1 if overwrite else 0]
+        cargs = [ctypes.c_int(v) for v in args]
+        L.p3dfft_setup(d, *[ctypes.byref(c) for c in cargs], ms)
+        self._raise_if_error()
+        self.active = True
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nxc, self.nyc, self.nzc = nxc or nx, nyc or ny, nzc or nz
+        self.stride1 = stride1
+        self.nxhpc = self.nxc // 2 + 1
+        self.memsize = tuple(ms)
+        return self.memsize
+
+    def get_dims(self, conf):
+        s = (ctypes.c_int * 3)(); e = (ctypes.c_int * 3)(); z = (ctypes.c_int * 3)()
+        c = ctypes.c_int(conf)
+        self.lib.p3dfft_get_dims(s, e, z, ctypes.byref(c))
+        return tuple(s), tuple(e), tuple(z)
+
+    def real_shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def spec_shape(self):
+        if self.stride1:
+            return (self.nxhpc, self.nyc, self.nzc)
+        return (self.nzc, self.nyc, self.nxhpc)
+
+    def ftran_r2c(self, a, op="fft"):
+        """Host arrays in and out (copied to/from HBM inside the call)."""
+        a = np.ascontiguousarray(a, dtype=self.rdtype).reshape(self.real_shape())
+        out = np.empty(self.spec_shape(), dtype=self.cdtype)
+        self.lib.p3dfft_ftran_r2c(a.ctypes.data, out.ctypes.data, _op(op))
+        self._raise_if_error()
+        return out
+
+    def btran_c2r(self, spec, op="tff"):
+        s = np.ascontiguousarray(spec, dtype=self.cdtype).reshape(self.spec_shape())
+        out = np.empty(self.real_shape(), dtype=self.rdtype)
+        self.lib.p3dfft_btran_c2r(s.ctypes.data, out.ctypes.data, _op(op))
+        self._raise_if_error()
+        return out
+
+    def cheby(self, a, Lz):
+        a = np.ascontiguousarray(a, dtype=self.rdtype).reshape(self.real_shape())
+        out = np.empty(self.spec_shape(), dtype=self.cdtype)
+        lz = (ctypes.c_double if self.precision == 8 else ctypes.c_float)(Lz)
+        self.lib.p3dfft_cheby(a.ctypes.data, out.ctypes.data, ctypes.byref(lz))
+        self._raise_if_error()
+        return out
+
+    def clean(self):
+        self.lib.p3dfft_clean()
+        self.active = False
+
+    def get_timers(self):
+        t = (ctypes.c_double * 12)()
+        self.lib.get_timers(t)
+        return list(t)
+
+    def set_timers(self):
+        self.lib.set_timers()
+
+    # -- device-pointer extensions -----------------------------------------
+    def ftran_r2c_dev(self, in_ptr, out_ptr, op="fft", stream=None):
+        rc = self.lib.p3dfft_ftran_r2c_dev(in_ptr, out_ptr, _op(op), stream)
+        if rc:
+            self._raise_if_error(force=True)
+
+    def btran_c2r_dev(self, in_ptr, out_ptr, op="tff", stream=None):
+        rc = self.lib.p3dfft_btran_c2r_dev(in_ptr, out_ptr, _op(op), stream)
+        if rc:
+            self._raise_if_error(force=True)
+
+    def cheby_dev(self, in_ptr, out_ptr, Lz, stream=None):
+        rc = self.lib.p3dfft_cheby_dev(in_ptr, out_ptr, Lz, stream)
+        if rc:
+            self._raise_if_error(force=True)
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self.lib, nbytes)
+
+    def real_bytes(self):
+        return self.nx * self.ny * self.nz * np.dtype(self.rdtype).itemsize
+
+    def spec_bytes(self):
+        return self.nxhpc * self.nyc * self.nzc * np.dtype(self.cdtype).itemsize
+
+    def synchronize(self):
+        if self.lib.p3dfft_amd_synchronize():
+            raise P3DFFTError("device synchronize failed")
+
+    def last_error(self):
+        v = self.lib.p3dfft_amd_last_error()
+        return v.decode() if v else ""
+
+    def _raise_if_error(self, force=False):
+        msg = self.last_error()
+        if msg or force:
+            self.lib.p3dfft_amd_clear_error()
+            raise P3DFFTError(msg or "p3dfft_amd call failed")
+
+
+def _op(op):
+    b = op.encode() if isinstance(op, str) else bytes(op)
+    return (b + b"\0\0\0")[:4]
+
+
+def device_count():
+    return load(8).p3dfft_amd_device_count()

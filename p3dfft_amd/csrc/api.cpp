@@ -1,0 +1,568 @@
+// api.cpp -- the C-ABI of libp3dfft_amd: P3DFFT's BIND(C) entry points for the
+// ftran_r2c / btran_c2r path, re-implemented over the HIP engine.
+//
+// State is module-global and there is one active setup at a time, as in the
+// reference (build/module.F90:101-176, build/setup.F90:130-135).  The single
+// GPU is rank 0 of a 1x1 processor grid: both transposes of the reference are
+// local (build/ftran.F90:539,595 and build/btran.F90:430,646 take the iproc=1 /
+// jproc=1 branches), so they are folded into the load/store indexing of the
+// neighbouring FFT kernels.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "p3dfft_amd_internal.hpp"
+
+using namespace p3amd;
+
+namespace p3amd {
+
+State g;
+
+static int g_err_mode = 0;  // 0: print and terminate (reference: MPI_Abort); 1: print and return
+static std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// Reference behaviour for fatal errors is "print, then MPI_Abort".
+bool fatal(const std::string& msg) {
+  fprintf(stderr, "%s\n", msg.c_str());
+  fflush(stderr);
+  g_last_error = msg;
+  if (g_err_mode == 0) std::exit(1);
+  return false;
+}
+
+// Messages that the reference prints before returning (no abort).
+void notice(const std::string& msg) {
+  fprintf(stderr, "%s\n", msg.c_str());
+  fflush(stderr);
+  g_last_error = msg;
+}
+
+// ---------------------------------------------------------------------------
+// timers: the reference's 12 accumulating wall-clock slots
+// (build/module.F90:106, 709-744), filled from HIP events recorded on the
+// stream around each stage and resolved lazily.
+// ---------------------------------------------------------------------------
+static hipEvent_t ev_get() {
+  hipEvent_t e;
+  if (!g.evpool.empty()) {
+    e = g.evpool.back();
+    g.evpool.pop_back();
+    return e;
+  }
+  check_hip(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
+void timers_resolve() {
+  for (auto& p : g.pend) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess)
+      g.timers[p.slot - 1] += ms * 1e-3;
+    g.evpool.push_back(p.a);
+    g.evpool.push_back(p.b);
+  }
+  g.pend.clear();
+}
+
+void tstart(int slot, hipStream_t s) {
+  if (!g.timing) return;
+  hipEvent_t a = ev_get();
+  check_hip(hipEventRecord(a, s), "hipEventRecord");
+  g.open[slot - 1] = a;
+}
+
+void tstop(int slot, hipStream_t s) {
+  if (!g.timing) return;
+  hipEvent_t b = ev_get();
+  check_hip(hipEventRecord(b, s), "hipEventRecord");
+  g.pend.push_back(Pending{slot, g.open[slot - 1], b});
+  if (g.pend.size() > 8192) timers_resolve();
+}
+
+// ---------------------------------------------------------------------------
+// decomposition (build/setup.F90:608-635, remainder to the last ranks)
+// ---------------------------------------------------------------------------
+void map_data_to_proc(int data, int proc, int* st, int* en, int* sz) {
+  int size = data / proc;
+  const int nu = data - size * proc;
+  const int nl = proc - nu;
+  st[0] = 1;
+  sz[0] = size;
+  en[0] = size;
+  for (int i = 1; i <= nl - 1; i++) {
+    st[i] = st[i - 1] + size;
+    sz[i] = size;
+    en[i] = en[i - 1] + size;
+  }
+  size = size + 1;
+  for (int i = nl; i <= proc - 1; i++) {
+    st[i] = en[i - 1] + 1;
+    sz[i] = size;
+    en[i] = en[i - 1] + size;
+  }
+  en[proc - 1] = data;
+  sz[proc - 1] = data - st[proc - 1] + 1;
+}
+
+// Sizes and local extents (build/setup.F90:148-176, 279-313, 580-603).
+bool compute_geometry(State& s, const int dims[2], int ipid, int jpid) {
+  s.p1 = dims[0];
+  s.p2 = dims[1];
+  s.ipid = ipid;
+  s.jpid = jpid;
+  s.nxh = s.nx / 2;
+  s.nxhp = s.nxh + 1;
+  s.nxhc = s.nxc / 2;
+  s.nxhpc = s.nxhc + 1;
+  s.nyh = s.ny / 2;
+  s.nzh = s.nz / 2;
+  s.nyhc = s.nyc / 2;
+  s.nzhc = s.nzc / 2;
+  s.nycph = (s.nyc + 1) / 2;
+  s.nzcph = (s.nzc + 1) / 2;
+  const int P = s.p1 > s.p2 ? s.p1 : s.p2;
+  std::vector<int> st(P), en(P), sz(P);
+  map_data_to_proc(s.nxhpc, s.p1, st.data(), en.data(), sz.data());
+  s.iist.assign(st.begin(), st.begin() + s.p1);
+  s.iisz.assign(sz.begin(), sz.begin() + s.p1);
+  s.iistart = st[ipid]; s.iiend = en[ipid]; s.iisize = sz[ipid];
+  map_data_to_proc(s.ny, s.p1, st.data(), en.data(), sz.data());
+  s.jist.assign(st.begin(), st.begin() + s.p1);
+  s.jisz.assign(sz.begin(), sz.begin() + s.p1);
+  s.jistart = st[ipid]; s.jiend = en[ipid]; s.jisize = sz[ipid];
+  map_data_to_proc(s.nyc, s.p2, st.data(), en.data(), sz.data());
+  s.jjst.assign(st.begin(), st.begin() + s.p2);
+  s.jjsz.assign(sz.begin(), sz.begin() + s.p2);
+  s.jjstart = st[jpid]; s.jjend = en[jpid]; s.jjsize = sz[jpid];
+  map_data_to_proc(s.nz, s.p2, st.data(), en.data(), sz.data());
+  s.kjst.assign(st.begin(), st.begin() + s.p2);
+  s.kjsz.assign(sz.begin(), sz.begin() + s.p2);
+  s.kjstart = st[jpid]; s.kjend = en[jpid]; s.kjsize = sz[jpid];
+  long long a = (long long)s.nz * s.jjsize * s.iisize;
+  long long b = (long long)s.ny * s.kjsize * s.iisize;
+  long long d = (long long)s.nx * s.jisize;
+  long long pad1 = 2 * (a > b ? a : b) - (long long)s.nx * s.jisize * s.kjsize;
+  if (pad1 <= 0) pad1 = 0;
+  long long padi = (pad1 % d != 0) ? pad1 / d + 1 : pad1 / d;
+  s.memsize[0] = s.nx;
+  s.memsize[1] = s.jisize;
+  s.memsize[2] = s.kjsize + (int)padi;
+  return true;
+}
+
+size_t real_elems() { return (size_t)g.nx * g.jisize * g.kjsize; }
+size_t spec_elems() { return (size_t)g.iisize * g.jjsize * g.nzc; }
+
+void ensure_staging() {
+  const size_t bytes = std::max(real_elems() * sizeof(real_t), spec_elems() * sizeof(cplx_t));
+  if (g.stage_bytes >= bytes) return;
+  if (g.stage_a) (void)hipFree(g.stage_a);
+  if (g.stage_b) (void)hipFree(g.stage_b);
+  g.stage_a = g.stage_b = nullptr;
+  g.stage_bytes = 0;
+  check_hip(hipMalloc(&g.stage_a, bytes), "hipMalloc(staging)");
+  check_hip(hipMalloc(&g.stage_b, bytes), "hipMalloc(staging)");
+  g.stage_bytes = bytes;
+}
+
+static int zkind(char c, bool* none) {
+  *none = false;
+  if (c == 't' || c == 'f') return COL_C2C;
+  if (c == 'c') return COL_DCT1;
+  if (c == 's') return COL_DST1;
+  if (c == 'n' || c == '0') { *none = true; return COL_COPY; }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// forward: X r2c -> Y c2c (per z-plane) -> Z op, build/ftran.F90:489-780
+// ---------------------------------------------------------------------------
+bool forward_dev(const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s) {
+  bool none;
+  const int kind = zkind((char)op[2], &none);
+  if (kind < 0) return fatal(std::string("Unknown transform type: ") + (char)op[2]);
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxhpc = g.nxhpc, nyc = g.nyc, nzc = g.nzc;
+  const int dny = ny - nyc, dnz = nz - nzc;
+  const long long nxy = (long long)nxhpc * nyc;
+  // X r2c into w1 (nxhpc, ny, nz); seg_copy_x folded into the store
+  tstart(5, s);
+  launch_x_r2c(in, g.w1, (long long)ny * nz, nx, nxhpc, nx, nxhpc, s);
+  tstop(5, s);
+  // Y c2c; seg_copy_y (keep y < nycph, then y + dny) folded into the store
+  const bool direct = (dnz == 0 && !g.stride1);
+  cplx_t* ydst = direct ? out : g.w2;
+  ColSpec y{};
+  y.in = g.w1; y.out = ydst;
+  y.si_in = 1; y.sl_in = nxhpc; y.so_in = (long long)nxhpc * ny;
+  y.si_out = 1; y.sl_out = nxhpc; y.so_out = nxy;
+  y.n_inner = nxhpc; y.n_outer = nz; y.n = ny; y.kind = COL_C2C; y.fwd = true;
+  y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
+  y.nst = nyc; y.st_lo = g.nycph; y.st_shift = dny;
+  tstart(7, s);
+  launch_col(y, s);
+  tstop(7, s);
+  // Z op; seg_copy_z (keep z < nzcph, then z + dnz) and the STRIDE1 layout
+  // folded into the store
+  tstart(8, s);
+  if (!(direct && none)) {
+    ColSpec z{};
+    z.in = ydst; z.out = out;
+    if (g.stride1) {
+      z.n_inner = nxhpc; z.n_outer = nyc;
+      z.si_in = 1; z.sl_in = nxy; z.so_in = nxhpc;
+      z.si_out = (long long)nzc * nyc; z.sl_out = 1; z.so_out = nzc;
+    } else {
+      z.n_inner = (int)nxy; z.n_outer = 1;
+      z.si_in = 1; z.sl_in = nxy; z.so_in = 0;
+      z.si_out = 1; z.sl_out = nxy; z.so_out = 0;
+    }
+    z.n = nz; z.kind = kind; z.fwd = true;
+    z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
+    z.nst = nzc; z.st_lo = g.nzcph; z.st_shift = dnz;
+    launch_col(z, s);
+  }
+  tstop(8, s);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// backward: Z op -> Y c2c -> X c2r, build/btran.F90:396-679
+// ---------------------------------------------------------------------------
+bool backward_dev(cplx_t* in, real_t* out, const unsigned char* op, hipStream_t s) {
+  bool none;
+  const int kind = zkind((char)op[0], &none);
+  if (kind < 0) return fatal(std::string(" Unknown transform type: ") + (char)op[0]);
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nxhpc = g.nxhpc, nyc = g.nyc, nzc = g.nzc;
+  const int dny = ny - nyc, dnz = nz - nzc;
+  const long long nxy = (long long)nxhpc * nyc;
+  cplx_t* zdst;
+  tstart(9, s);
+  if (!g.stride1 && g.ow && nz == nzc) {
+    // Z op in place on the caller's array (btran.F90:525-550): the input
+    // spectrum is overwritten, as in the reference
+    if (!none) {
+      ColSpec z{};
+      z.in = in; z.out = in;
+      z.n_inner = (int)nxy; z.n_outer = 1;
+      z.si_in = 1; z.sl_in = nxy; z.si_out = 1; z.sl_out = nxy;
+      z.n = nz; z.kind = kind; z.fwd = false;
+      z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
+      z.nst = nz; z.st_lo = nz; z.st_shift = 0;
+      launch_col(z, s);
+    }
+    zdst = in;
+  } else {
+    // seg_copy_z + seg_zero_z folded into the load (btran.F90:556-558;
+    // STRIDE1: reorder.F90:208-245 keeps z >= nzcph+dnz)
+    ColSpec z{};
+    z.in = in; z.out = g.w2;
+    if (g.stride1) {
+      z.n_inner = nxhpc; z.n_outer = nyc;
+      z.si_in = (long long)nzc * nyc; z.sl_in = 1; z.so_in = nzc;
+      z.si_out = 1; z.sl_out = nxy; z.so_out = nxhpc;
+    } else {
+      z.n_inner = (int)nxy; z.n_outer = 1;
+      z.si_in = 1; z.sl_in = nxy; z.so_in = 0;
+      z.si_out = 1; z.sl_out = nxy; z.so_out = 0;
+    }
+    z.n = nz; z.kind = kind; z.fwd = false;
+    z.nld = nz; z.ld_lo = g.nzcph; z.ld_hi = g.stride1 ? g.nzcph + dnz : nz - g.nzcph; z.ld_shift = dnz;
+    z.nst = nz; z.st_lo = nz; z.st_shift = 0;
+    launch_col(z, s);
+    zdst = g.w2;
+  }
+  tstop(9, s);
+  // Y c2c backward; seg_copy_y + seg_zero_y folded into the load
+  ColSpec y{};
+  y.in = zdst; y.out = g.w1;
+  y.si_in = 1; y.sl_in = nxhpc; y.so_in = nxy;
+  y.si_out = 1; y.sl_out = nxhpc; y.so_out = (long long)nxhpc * ny;
+  y.n_inner = nxhpc; y.n_outer = nz; y.n = ny; y.kind = COL_C2C; y.fwd = false;
+  y.nld = ny; y.ld_lo = g.nycph; y.ld_hi = g.stride1 ? g.nycph + dny : ny - g.nycph; y.ld_shift = dny;
+  y.nst = ny; y.st_lo = ny; y.st_shift = 0;
+  tstart(10, s);
+  launch_col(y, s);
+  tstop(10, s);
+  // X c2r; seg_copy_x + seg_zero_x (x > nxhpc) folded into the load
+  tstart(12, s);
+  launch_x_c2r(g.w1, out, (long long)ny * nz, nxhpc, nx, nx, nxhpc, s);
+  tstop(12, s);
+  return true;
+}
+
+bool cheby_dev(const real_t* in, cplx_t* out, double Lz, hipStream_t s) {
+  static const unsigned char ffc[4] = {'f', 'f', 'c', 0};
+  if (g.nzc < 2) return fatal("p3dfft_cheby: nzc must be at least 2");
+  if (!forward_dev(in, out, ffc, s)) return false;
+  const double scale = 1.0 / ((double)g.nx * (double)g.ny * (double)(g.nzc - 1));
+  const long long ncol = (long long)g.nxhpc * g.nyc;
+  if (g.stride1) launch_cheby(out, ncol, g.nzc, g.nzc, 1, scale, 4.0 / Lz, s);
+  else launch_cheby(out, ncol, g.nzc, 1, ncol, scale, 4.0 / Lz, s);
+  return true;
+}
+
+bool require_setup() {
+  if (!g.set) {
+    notice(" P3DFFT error: call setup before other routines");
+    return false;
+  }
+  return true;
+}
+
+void release_state() {
+  timers_resolve();
+  for (auto e : g.evpool) (void)hipEventDestroy(e);
+  g.evpool.clear();
+  if (g.w1) (void)hipFree(g.w1);
+  if (g.w2) (void)hipFree(g.w2);
+  if (g.stage_a) (void)hipFree(g.stage_a);
+  if (g.stage_b) (void)hipFree(g.stage_b);
+  g.w1 = g.w2 = nullptr;
+  g.stage_a = g.stage_b = nullptr;
+  g.stage_bytes = 0;
+  g.set = false;
+}
+
+}  // namespace p3amd
+
+// ===========================================================================
+// exported entry points
+// ===========================================================================
+#define GUARD_BEGIN try {
+#define GUARD_END                                  \
+  }                                                \
+  catch (const Error& e) {                         \
+    fatal(std::string("p3dfft_amd: ") + e.msg);    \
+  }
+
+extern "C" {
+
+// build/setup.F90:83-104 (BIND(C) p3dfft_setup) and 107-605
+void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int* nyc, int* nzc, int* ow,
+                  int* memsize) {
+  (void)comm;  // the single GPU is rank 0 of a one-rank communicator
+  GUARD_BEGIN
+  if (*nx <= 0 || *ny <= 0 || *nz <= 0) {
+    fatal(" Invalid dimensions :" + std::to_string(*nx) + " " + std::to_string(*ny) + " " + std::to_string(*nz));
+    return;
+  }
+  if (g.set) {
+    fatal(" P3DFFT Setup error: the problem is already initialized. \n"
+          " Currently multiple setups not supported.\n"
+          " Quit the library using p3dfft_clean before initializing another setup");
+    return;
+  }
+  if (dims[0] <= 0 || dims[1] <= 0 || dims[0] * dims[1] != 1) {
+    fatal(" Invalid processor geometry: " + std::to_string(dims[0]) + " " + std::to_string(dims[1]) +
+          " for 1 tasks");
+    return;
+  }
+  if (*nxc < 1 || *nxc > *nx || *nyc < 1 || *nyc > *ny || *nzc < 1 || *nzc > *nz) {
+    fatal(" Invalid cut dimensions :" + std::to_string(*nxc) + " " + std::to_string(*nyc) + " " +
+          std::to_string(*nzc));
+    return;
+  }
+  for (int i = 0; i < 12; i++) g.timers[i] = 0.0;
+  g.nx = *nx; g.ny = *ny; g.nz = *nz;
+  g.nxc = *nxc; g.nyc = *nyc; g.nzc = *nzc;
+  g.ow = (*ow != 0);
+  g.stride1 = g.stride1_next;
+  compute_geometry(g, dims, 0, 0);
+  if (g.stride1) printf(" Using stride-1 layout\n");
+  const int lim = max_len();
+  if (g.nx > lim || g.ny > lim || 2 * (g.nz + 1) > lim) {
+    fatal(" p3dfft_amd: grid dimension exceeds the supported transform length " + std::to_string(lim));
+    return;
+  }
+  check_hip(hipMalloc(&g.w1, (size_t)g.nxhpc * g.ny * g.nz * sizeof(cplx_t)), "hipMalloc(work)");
+  check_hip(hipMalloc(&g.w2, (size_t)g.nxhpc * g.nyc * g.nz * sizeof(cplx_t)), "hipMalloc(work)");
+  // build the 1D plans (twiddles are generated on the GPU here, once)
+  get_len(g.nx);
+  get_len(g.ny);
+  get_len(g.nz);
+  g.set = true;
+  memsize[0] = g.memsize[0];
+  memsize[1] = g.memsize[1];
+  memsize[2] = g.memsize[2];
+  GUARD_END
+}
+
+// build/module.F90:214-273
+void p3dfft_get_dims(int* start, int* end, int* size, int* conf) {
+  if (!require_setup()) return;
+  if (*conf == 1) {
+    start[0] = 1; end[0] = g.nx; size[0] = g.nx;
+    start[1] = g.jistart; end[1] = g.jiend; size[1] = g.jisize;
+    start[2] = g.kjstart; end[2] = g.kjend; size[2] = g.kjsize;
+  } else if (*conf == 2) {
+    if (g.stride1) {
+      start[2] = g.iistart; end[2] = g.iiend; size[2] = g.iisize;
+      start[1] = g.jjstart; end[1] = g.jjend; size[1] = g.jjsize;
+      start[0] = 1; end[0] = g.nzc; size[0] = g.nzc;
+    } else {
+      start[0] = g.iistart; end[0] = g.iiend; size[0] = g.iisize;
+      start[1] = g.jjstart; end[1] = g.jjend; size[1] = g.jjsize;
+      start[2] = 1; end[2] = g.nzc; size[2] = g.nzc;
+    }
+  } else if (*conf == 3) {
+    for (int i = 0; i < 3; i++) { start[i] = 0; end[i] = g.memsize[i]; size[i] = g.memsize[i]; }
+  }
+}
+
+// build/ftran.F90:469-485 (host arrays; copied to and from HBM inside the call)
+void p3dfft_ftran_r2c(real_t* A, real_t* B, unsigned char* op) {
+  if (!require_setup()) return;
+  GUARD_BEGIN
+  ensure_staging();
+  check_hip(hipMemcpy(g.stage_a, A, real_elems() * sizeof(real_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  if (!forward_dev((const real_t*)g.stage_a, (cplx_t*)g.stage_b, op, g.stream)) return;
+  check_hip(hipMemcpy(B, g.stage_b, spec_elems() * sizeof(cplx_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  GUARD_END
+}
+
+// build/btran.F90:377-392
+void p3dfft_btran_c2r(real_t* A, real_t* B, unsigned char* op) {
+  if (!require_setup()) return;
+  GUARD_BEGIN
+  ensure_staging();
+  check_hip(hipMemcpy(g.stage_a, A, spec_elems() * sizeof(cplx_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  if (!backward_dev((cplx_t*)g.stage_a, (real_t*)g.stage_b, op, g.stream)) return;
+  check_hip(hipMemcpy(B, g.stage_b, real_elems() * sizeof(real_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  GUARD_END
+}
+
+// build/ftran.F90:359-379
+void p3dfft_cheby(real_t* A, real_t* B, real_t* Lz) {
+  if (!require_setup()) return;
+  GUARD_BEGIN
+  ensure_staging();
+  check_hip(hipMemcpy(g.stage_a, A, real_elems() * sizeof(real_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  if (!cheby_dev((const real_t*)g.stage_a, (cplx_t*)g.stage_b, (double)*Lz, g.stream)) return;
+  check_hip(hipMemcpy(B, g.stage_b, spec_elems() * sizeof(cplx_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  GUARD_END
+}
+
+// build/module.F90:301-424
+void p3dfft_clean(void) {
+  GUARD_BEGIN
+  release_state();
+  GUARD_END
+}
+
+// build/module.F90:709-744
+void get_timers(double* t) {
+  GUARD_BEGIN
+  timers_resolve();
+  GUARD_END
+  for (int i = 0; i < 12; i++) t[i] = g.timers[i];
+}
+
+void set_timers(void) {
+  GUARD_BEGIN
+  timers_resolve();
+  GUARD_END
+  for (int i = 0; i < 12; i++) g.timers[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// extensions (no counterpart in the reference)
+// ---------------------------------------------------------------------------
+
+// Device-pointer variants: in/out are HBM addresses, the work is queued on
+// `stream` (a hipStream_t, or null for the default stream) and the call
+// returns without synchronising.
+int p3dfft_ftran_r2c_dev(const real_t* in, real_t* out, const unsigned char* op, void* stream) {
+  if (!require_setup()) return -1;
+  GUARD_BEGIN
+  return forward_dev(in, (cplx_t*)out, op, (hipStream_t)stream) ? 0 : -1;
+  GUARD_END
+  return -1;
+}
+
+int p3dfft_btran_c2r_dev(real_t* in, real_t* out, const unsigned char* op, void* stream) {
+  if (!require_setup()) return -1;
+  GUARD_BEGIN
+  return backward_dev((cplx_t*)in, out, op, (hipStream_t)stream) ? 0 : -1;
+  GUARD_END
+  return -1;
+}
+
+int p3dfft_cheby_dev(const real_t* in, real_t* out, real_t Lz, void* stream) {
+  if (!require_setup()) return -1;
+  GUARD_BEGIN
+  return cheby_dev(in, (cplx_t*)out, (double)Lz, (hipStream_t)stream) ? 0 : -1;
+  GUARD_END
+  return -1;
+}
+
+// Select the (Z,Y,X) complex layout for the next setup (the reference's
+// compile-time STRIDE1, build/module.F90:245-265).
+void p3dfft_amd_set_stride1(int on) { g.stride1_next = (on != 0); }
+
+// 0: fatal errors print and terminate the process (the reference's MPI_Abort);
+// 1: they print, record the message and return.
+void p3dfft_amd_set_error_mode(int mode) { g_err_mode = mode; }
+const char* p3dfft_amd_last_error(void) { return g_last_error.c_str(); }
+void p3dfft_amd_clear_error(void) { g_last_error.clear(); }
+
+// Stage timers on/off (default on).
+void p3dfft_amd_set_timing(int on) {
+  if (!on) timers_resolve();
+  g.timing = (on != 0);
+}
+
+int p3dfft_amd_precision(void) { return (int)sizeof(real_t); }
+
+// Device memory helpers for callers without their own HIP runtime bindings.
+void* p3dfft_amd_malloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    set_error("hipMalloc failed");
+    return nullptr;
+  }
+  return p;
+}
+int p3dfft_amd_free(void* p) { return hipFree(p) == hipSuccess ? 0 : -1; }
+int p3dfft_amd_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+int p3dfft_amd_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+int p3dfft_amd_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? 0 : -1;
+}
+int p3dfft_amd_memset(void* dst, int value, size_t bytes) {
+  return hipMemset(dst, value, bytes) == hipSuccess ? 0 : -1;
+}
+int p3dfft_amd_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+int p3dfft_amd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// Events on the null stream, for callers timing their own regions.
+void* p3dfft_amd_event_create(void) {
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return (void*)e;
+}
+int p3dfft_amd_event_record(void* e, void* stream) {
+  return hipEventRecord((hipEvent_t)e, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+float p3dfft_amd_event_elapsed_ms(void* a, void* b) {
+  float ms = -1.f;
+  if (hipEventSynchronize((hipEvent_t)b) != hipSuccess) return -1.f;
+  if (hipEventElapsedTime(&ms, (hipEvent_t)a, (hipEvent_t)b) != hipSuccess) return -1.f;
+  return ms;
+}
+int p3dfft_amd_event_destroy(void* e) { return hipEventDestroy((hipEvent_t)e) == hipSuccess ? 0 : -1; }
+
+}  // extern "C"

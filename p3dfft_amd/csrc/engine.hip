@@ -1,0 +1,703 @@
+// engine.hip -- gfx950 kernels of the 3D FFT path and their launchers.
+//
+//  k_x_r2c / k_x_c2r : transforms along the contiguous X axis.  Two real rows
+//                      are packed into one complex row (x_a + i x_b) and run
+//                      through one complex FFT; the two half spectra are
+//                      separated (r2c) or merged (c2r) in LDS.  Replaces the
+//                      FFTW r2c/c2r plans of build/fft_exec.F90:495-540,298-342.
+//  k_col             : transform along a strided axis (Y or Z) of a 3D view.
+//                      A workgroup owns TX adjacent inner columns of one outer
+//                      index; loads are coalesced along the inner axis (or along
+//                      the transform axis for the STRIDE1 layout), the pruning
+//                      index maps and zero fills of build/module.F90:502-706 are
+//                      applied in the load/store indexing, and DCT-I / DST-I are
+//                      run as complex FFTs of the even / odd extension built in
+//                      LDS (build/fft_exec.F90:646-701, 866-921).
+//  k_gen_tw          : FP64 twiddle generation (sincospi), stored as double2 or
+//                      as hi+lo float pairs.
+//  k_cheby           : Chebyshev derivative recurrence (build/ftran.F90:408-451)
+//                      in FP64, one thread per (x,y) column.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "engine.hpp"
+
+namespace p3amd {
+
+void check_hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error{std::string("HIP error in ") + what + ": " + hipGetErrorString(e), -3};
+}
+
+typedef TwT<real_t>::type Tw;
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Remap the flat
+// workgroup id so that each XCD owns a contiguous range of tiles: adjacent
+// tiles (which share partial cache lines at their edges) then meet in the same
+// L2.  The map is a bijection on [0, nwg).
+__device__ __forceinline__ int xcd_tile(int bid, int nwg) {
+  const int q = nwg >> 3, rm = nwg & 7;
+  const int x = bid & 7, slot = bid >> 3;
+  return x * q + (x < rm ? x : rm) + slot;
+}
+
+// ---------------------------------------------------------------------------
+// Stockham stages over an LDS tile.  COL layout: element e of sequence b at
+// lds[e*B + b] (B sequences side by side, sequence index fastest).  ROW layout:
+// lds[b*n + e].
+// ---------------------------------------------------------------------------
+template <class T, int R, bool FWD, bool COL, int NT, int EPT>
+__device__ __forceinline__ void stage_r(cpx<T>* lds, const LenDesc& L, int s, int B, FDiv dB,
+                                        const typename TwT<T>::type* tw) {
+  constexpr int IT = EPT / R;
+  static_assert(IT >= 1, "radix larger than the per-thread element budget");
+  const int n = L.n, nR = n / R, Ns = L.ns[s];
+  const int total = B * nR;
+  cpx<T> v[IT][R];
+#pragma unroll
+  for (int m = 0; m < IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if (it < total) {
+      int b, j;
+      if (COL) { j = fdiv(it, dB); b = it - j * B; }
+      else { b = fdiv(it, L.dnr[s]); j = it - b * nR; }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int e = j + r * nR;
+        v[m][r] = lds[COL ? e * B + b : b * n + e];
+      }
+      if (s > 0) {
+        const int k = j - fdiv(j, L.dns[s]) * Ns;
+        const typename TwT<T>::type* w = tw + L.twoff[s] + k * (R - 1);
+#pragma unroll
+        for (int r = 1; r < R; r++) v[m][r] = twmul<FWD>(v[m][r], w[r - 1]);
+      }
+      dft<R, FWD>(v[m]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if (it < total) {
+      int b, j;
+      if (COL) { j = fdiv(it, dB); b = it - j * B; }
+      else { b = fdiv(it, L.dnr[s]); j = it - b * nR; }
+      const int k = j - fdiv(j, L.dns[s]) * Ns;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int e = base + r * Ns;
+        lds[COL ? e * B + b : b * n + e] = v[m][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Any radix (used for prime factors above 13): one output per work item,
+// y = sum_r x_r W_{Ns R}^{r (k + q Ns)}, twiddles from the full table W_n^e.
+template <class T, bool FWD, bool COL, int NT, int EPT>
+__device__ __forceinline__ void stage_g(cpx<T>* lds, const LenDesc& L, int s, int B, FDiv dB,
+                                        const typename TwT<T>::type* tw) {
+  const int n = L.n, R = L.radix[s], nR = n / R, Ns = L.ns[s], M = Ns * R, step = n / M;
+  const int total = B * n;
+  const typename TwT<T>::type* wf = tw + L.fulloff;
+  cpx<T> v[EPT];
+#pragma unroll
+  for (int m = 0; m < EPT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if (it < total) {
+      int b, jq;
+      if (COL) { jq = fdiv(it, dB); b = it - jq * B; }
+      else { b = fdiv(it, L.dn); jq = it - b * n; }
+      const int q = fdiv(jq, L.dnr[s]);
+      const int j = jq - q * nR;
+      const int k = j - fdiv(j, L.dns[s]) * Ns;
+      const int inc = k + q * Ns;
+      int e = 0;
+      cpx<T> acc = mk<T>(0, 0);
+      for (int r = 0; r < R; r++) {
+        const int ee = j + r * nR;
+        acc = add(acc, twmul<FWD>(lds[COL ? ee * B + b : b * n + ee], wf[e * step]));
+        e += inc;
+        if (e >= M) e -= M;
+      }
+      v[m] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < EPT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if (it < total) {
+      int b, jq;
+      if (COL) { jq = fdiv(it, dB); b = it - jq * B; }
+      else { b = fdiv(it, L.dn); jq = it - b * n; }
+      const int q = fdiv(jq, L.dnr[s]);
+      const int j = jq - q * nR;
+      const int k = j - fdiv(j, L.dns[s]) * Ns;
+      const int e = (j - k) * R + k + q * Ns;
+      lds[COL ? e * B + b : b * n + e] = v[m];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, bool FWD, bool COL, int NT, int EPT>
+__device__ void run_fft(cpx<T>* lds, const LenDesc& L, int B, FDiv dB, const typename TwT<T>::type* tw) {
+  for (int s = 0; s < L.nst; s++) {
+    switch (L.radix[s]) {
+      case 2: stage_r<T, 2, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      case 3: stage_r<T, 3, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      case 4: stage_r<T, 4, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      case 5: stage_r<T, 5, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      case 7: stage_r<T, 7, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      case 8: stage_r<T, 8, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      case 11: stage_r<T, 11, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      case 13: stage_r<T, 13, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      default: stage_g<T, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+    }
+  }
+}
+
+// copy the twiddle table into LDS behind the data tile
+template <class T, int NT>
+__device__ __forceinline__ const typename TwT<T>::type* stage_twiddles(unsigned char* smem, size_t off,
+                                                                      const typename TwT<T>::type* tw,
+                                                                      int count) {
+  typedef typename TwT<T>::type TwL;
+  TwL* tl = (TwL*)(smem + off);
+  for (int i = threadIdx.x; i < count; i += NT) tl[i] = tw[i];
+  return tl;
+}
+
+// ---------------------------------------------------------------------------
+// X kernels
+// ---------------------------------------------------------------------------
+struct XArgs {
+  const void* in;
+  void* out;
+  long long nrows, si_row, so_row;
+  int nxo;     // r2c: kept modes; c2r: valid input modes
+  int B;       // row pairs per workgroup
+  FDiv dnxo;
+  LenDesc L;
+  const void* tw;
+  int tw_lds;  // twiddles staged in LDS
+  int tw_off;  // byte offset of the staged twiddles
+};
+
+extern __shared__ __align__(16) unsigned char g_smem[];
+
+template <class T, int NT, int EPT>
+__global__ __launch_bounds__(NT) void k_x_r2c(XArgs a) {
+  typedef typename TwT<T>::type TwL;
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const int n = a.L.n, B = a.B;
+  const TwL* tw = (const TwL*)a.tw;
+  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
+  const T* in = (const T*)a.in;
+  for (int q = threadIdx.x; q < B * n; q += NT) {
+    const int b = fdiv(q, a.L.dn), j = q - b * n;
+    const long long ra = 2 * (pair0 + b), rb = ra + 1;
+    const T xa = ra < a.nrows ? in[ra * a.si_row + j] : (T)0;
+    const T xb = rb < a.nrows ? in[rb * a.si_row + j] : (T)0;
+    lds[b * n + j] = mk<T>(xa, xb);
+  }
+  __syncthreads();
+  run_fft<T, true, false, NT, EPT>(lds, a.L, B, FDiv{0, 1}, tw);
+  cpx<T>* out = (cpx<T>*)a.out;
+  const T h = (T)0.5;
+  for (int q = threadIdx.x; q < B * a.nxo; q += NT) {
+    const int b = fdiv(q, a.dnxo), k = q - b * a.nxo;
+    const cpx<T> zk = lds[b * n + k], zn = lds[b * n + (k == 0 ? 0 : n - k)];
+    const long long ra = 2 * (pair0 + b), rb = ra + 1;
+    if (ra < a.nrows) out[ra * a.so_row + k] = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h);
+    if (rb < a.nrows) out[rb * a.so_row + k] = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
+  }
+}
+
+template <class T, int NT, int EPT>
+__global__ __launch_bounds__(NT) void k_x_c2r(XArgs a) {
+  typedef typename TwT<T>::type TwL;
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const int n = a.L.n, B = a.B, h = n / 2;
+  const TwL* tw = (const TwL*)a.tw;
+  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
+  const cpx<T>* in = (const cpx<T>*)a.in;
+  for (int q = threadIdx.x; q < B * n; q += NT) {
+    const int b = fdiv(q, a.L.dn), k = q - b * n;
+    const long long ra = 2 * (pair0 + b), rb = ra + 1;
+    const int kk = k <= h ? k : n - k;
+    cpx<T> A = mk<T>(0, 0), Bv = mk<T>(0, 0);
+    if (kk < a.nxo) {
+      if (ra < a.nrows) A = in[ra * a.si_row + kk];
+      if (rb < a.nrows) Bv = in[rb * a.si_row + kk];
+    }
+    // FFTW c2r: the imaginary parts of k=0 and k=n/2 (n even) do not contribute
+    if (kk == 0 || 2 * kk == n) { A.y = 0; Bv.y = 0; }
+    lds[b * n + k] = (k == kk) ? mk<T>(A.x - Bv.y, A.y + Bv.x) : mk<T>(A.x + Bv.y, Bv.x - A.y);
+  }
+  __syncthreads();
+  run_fft<T, false, false, NT, EPT>(lds, a.L, B, FDiv{0, 1}, tw);
+  T* out = (T*)a.out;
+  for (int q = threadIdx.x; q < B * n; q += NT) {
+    const int b = fdiv(q, a.L.dn), j = q - b * n;
+    const long long ra = 2 * (pair0 + b), rb = ra + 1;
+    const cpx<T> z = lds[b * n + j];
+    if (ra < a.nrows) out[ra * a.so_row + j] = z.x;
+    if (rb < a.nrows) out[rb * a.so_row + j] = z.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// column kernel
+// ---------------------------------------------------------------------------
+struct ColArgs {
+  const void* in;
+  void* out;
+  long long si_in, sl_in, so_in, si_out, sl_out, so_out;
+  int n_inner, n_outer, tiles;
+  int TX;
+  FDiv dTX;
+  int kind;
+  int nld, ld_lo, ld_hi, ld_shift;
+  int nst, st_lo, st_shift;
+  int ld_lenfast, st_lenfast;
+  FDiv dnld, dnst;
+  int nrows_lds;  // rows of the LDS tile
+  LenDesc L;
+  const void* tw;
+  int tw_lds, tw_off;
+};
+
+template <class T, bool FWD, int NT, int EPT>
+__global__ __launch_bounds__(NT) void k_col(ColArgs a) {
+  typedef typename TwT<T>::type TwL;
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const int TX = a.TX;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int o = t / a.tiles;
+  const int ti = t - o * a.tiles;
+  const long long i0 = (long long)ti * TX;
+  const int nb = (int)((long long)a.n_inner - i0 < TX ? (long long)a.n_inner - i0 : TX);
+  const cpx<T>* in = (const cpx<T>*)a.in + i0 * a.si_in + (long long)o * a.so_in;
+  cpx<T>* out = (cpx<T>*)a.out + i0 * a.si_out + (long long)o * a.so_out;
+  const TwL* tw = (const TwL*)a.tw;
+  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  const int off = (a.kind == COL_DST1) ? 1 : 0;
+  for (int q = threadIdx.x; q < TX * a.nld; q += NT) {
+    int b, e;
+    if (a.ld_lenfast) { b = fdiv(q, a.dnld); e = q - b * a.nld; }
+    else { e = fdiv(q, a.dTX); b = q - e * TX; }
+    cpx<T> v = mk<T>(0, 0);
+    if (b < nb) {
+      const int src = e < a.ld_lo ? e : (e >= a.ld_hi ? e - a.ld_shift : -1);
+      if (src >= 0) v = in[(long long)b * a.si_in + (long long)src * a.sl_in];
+    }
+    lds[(e + off) * TX + b] = v;
+  }
+  __syncthreads();
+  if (a.kind == COL_DCT1) {
+    // even extension of length m = 2(nld-1): row e <- row m-e
+    const int m = a.L.n;
+    for (int q = threadIdx.x; q < TX * (m - a.nld); q += NT) {
+      const int r = fdiv(q, a.dTX), b = q - r * TX, e = a.nld + r;
+      lds[e * TX + b] = lds[(m - e) * TX + b];
+    }
+    __syncthreads();
+  } else if (a.kind == COL_DST1) {
+    // odd extension of length m = 2(nld+1): rows 0 and nld+1 zero, row e <- -row m-e
+    const int m = a.L.n;
+    for (int q = threadIdx.x; q < TX * (m - a.nld - 1); q += NT) {
+      const int r = fdiv(q, a.dTX), b = q - r * TX, e = a.nld + 1 + r;
+      if (e == a.nld + 1) lds[e * TX + b] = mk<T>(0, 0);
+      else { const cpx<T> u = lds[(m - e) * TX + b]; lds[e * TX + b] = mk<T>(-u.x, -u.y); }
+    }
+    for (int b = threadIdx.x; b < TX; b += NT) lds[b] = mk<T>(0, 0);
+    __syncthreads();
+  }
+  if (a.kind != COL_COPY) run_fft<T, FWD, true, NT, EPT>(lds, a.L, TX, a.dTX, tw);
+  for (int q = threadIdx.x; q < TX * a.nst; q += NT) {
+    int b, e;
+    if (a.st_lenfast) { b = fdiv(q, a.dnst); e = q - b * a.nst; }
+    else { e = fdiv(q, a.dTX); b = q - e * TX; }
+    if (b < nb) {
+      const int p = e < a.st_lo ? e : e + a.st_shift;
+      cpx<T> v;
+      if (a.kind == COL_DST1) { const cpx<T> u = lds[(p + 1) * TX + b]; v = mk<T>(-u.y, u.x); }
+      else v = lds[p * TX + b];
+      out[(long long)b * a.si_out + (long long)e * a.sl_out] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+template <class TwL>
+__global__ void k_gen_tw(const int2* idx, int count, TwL* out);
+
+template <>
+__global__ void k_gen_tw<double2>(const int2* idx, int count, double2* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int2 em = idx[i];
+  double s, c;
+  sincospi(2.0 * (double)em.x / (double)em.y, &s, &c);
+  out[i] = make_double2(c, -s);
+}
+
+template <>
+__global__ void k_gen_tw<float4>(const int2* idx, int count, float4* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int2 em = idx[i];
+  double s, c;
+  sincospi(2.0 * (double)em.x / (double)em.y, &s, &c);
+  s = -s;
+  const float ch = (float)c, sh = (float)s;
+  out[i] = make_float4(ch, sh, (float)(c - (double)ch), (float)(s - (double)sh));
+}
+
+template <class T>
+__global__ void k_cheby(cpx<T>* o, long long ncol, int N, long long cs, long long ks, double scale,
+                        double L) {
+  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (c >= ncol || N < 2) return;
+  cpx<T>* p = o + c * cs;
+  // element k (1-based) is p[(k-1)*ks]; the whole recurrence runs in FP64
+  cpx<T> t = p[(long long)(N - 1) * ks];
+  const double aNx = t.x * scale, aNy = t.y * scale;
+  t = p[(long long)(N - 2) * ks];
+  double oldx = t.x * scale, oldy = t.y * scale;
+  const double d1x = L * (double)(N - 1) * aNx * 0.5, d1y = L * (double)(N - 1) * aNy * 0.5;
+  p[(long long)(N - 1) * ks] = mk<T>((T)0, (T)0);
+  if (N == 2) {
+    p[0] = mk<T>((T)(d1x * 0.5), (T)(d1y * 0.5));
+    return;
+  }
+  p[(long long)(N - 2) * ks] = mk<T>((T)d1x, (T)d1y);
+  double p2x = 0, p2y = 0, p1x = d1x, p1y = d1y;
+  for (int k = N - 2; k >= 1; k--) {
+    t = p[(long long)(k - 1) * ks];
+    const double akx = t.x * scale, aky = t.y * scale;
+    double dkx = L * (double)k * oldx + p2x, dky = L * (double)k * oldy + p2y;
+    p2x = p1x; p2y = p1y;
+    p1x = dkx; p1y = dky;
+    if (k == 1) { dkx *= 0.5; dky *= 0.5; }
+    p[(long long)(k - 1) * ks] = mk<T>((T)dkx, (T)dky);
+    oldx = akx; oldy = aky;
+  }
+}
+
+template <class T>
+__global__ void k_scale(cpx<T>* p, long long n, double s) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) { cpx<T> v = p[i]; p[i] = mk<T>((T)(v.x * s), (T)(v.y * s)); }
+}
+
+// ---------------------------------------------------------------------------
+// host side: length plans
+// ---------------------------------------------------------------------------
+static FDiv make_fdiv(int d) {
+  FDiv f;
+  f.d = d;
+  f.m = ((1ULL << 32) + (uint64_t)d - 1) / (uint64_t)d;
+  return f;
+}
+
+// stage radices: radix 8 as far as possible (two 4s instead of an 8 and a 2),
+// then 3, 5, 7, 11, 13, then any remaining prime (generic stage)
+static std::vector<int> factorize(int n) {
+  std::vector<int> f;
+  int m = n, a = 0;
+  while (m % 2 == 0) { m /= 2; a++; }
+  int n8 = a / 3, rem = a % 3;
+  if (rem == 1 && n8 > 0) { n8--; f.push_back(4); f.push_back(4); }
+  else if (rem == 1) f.push_back(2);
+  else if (rem == 2) f.push_back(4);
+  for (int i = 0; i < n8; i++) f.insert(f.begin(), 8);
+  for (int p : {3, 5, 7, 11, 13})
+    while (m % p == 0) { f.push_back(p); m /= p; }
+  for (int p = 17; (long long)p * p <= m; p += 2)
+    while (m % p == 0) { f.push_back(p); m /= p; }
+  if (m > 1) f.push_back(m);
+  return f;
+}
+
+static bool is_templated(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 11 || r == 13;
+}
+
+static std::mutex g_len_mu;
+static std::map<int, Len1D> g_lens;
+
+const Len1D& get_len(int n) {
+  std::lock_guard<std::mutex> lk(g_len_mu);
+  auto it = g_lens.find(n);
+  if (it != g_lens.end()) return it->second;
+  Len1D L;
+  std::memset(&L.d, 0, sizeof(L.d));
+  L.d.n = n;
+  std::vector<int> f = n > 1 ? factorize(n) : std::vector<int>{};
+  if ((int)f.size() > kMaxStages) throw Error{"transform length has too many factors", -1};
+  std::vector<int2> idx;
+  int ns = 1;
+  for (size_t s = 0; s < f.size(); s++) {
+    const int R = f[s];
+    L.d.radix[s] = R;
+    L.d.ns[s] = ns;
+    L.d.twoff[s] = (int)idx.size();
+    L.d.dns[s] = make_fdiv(ns);
+    L.d.dnr[s] = make_fdiv(n / R);
+    if (!is_templated(R)) L.has_generic = true;
+    if (s > 0 && is_templated(R))
+      for (int k = 0; k < ns; k++)
+        for (int r = 1; r < R; r++) idx.push_back(make_int2(k * r, ns * R));
+    ns *= R;
+  }
+  L.d.nst = (int)f.size();
+  L.d.dn = make_fdiv(n);
+  L.d.fulloff = (int)idx.size();
+  if (L.has_generic)
+    for (int e = 0; e < n; e++) idx.push_back(make_int2(e, n));
+  L.d.twcount = (int)idx.size();
+  if (!idx.empty()) {
+    int2* didx = nullptr;
+    check_hip(hipMalloc(&didx, idx.size() * sizeof(int2)), "hipMalloc(twiddle index)");
+    check_hip(hipMemcpy(didx, idx.data(), idx.size() * sizeof(int2), hipMemcpyHostToDevice), "hipMemcpy");
+    check_hip(hipMalloc(&L.tw, idx.size() * sizeof(Tw)), "hipMalloc(twiddles)");
+    const int cnt = (int)idx.size();
+    k_gen_tw<Tw><<<(cnt + 255) / 256, 256>>>(didx, cnt, L.tw);
+    check_hip(hipGetLastError(), "k_gen_tw launch");
+    check_hip(hipDeviceSynchronize(), "k_gen_tw");
+    check_hip(hipFree(didx), "hipFree");
+  }
+  auto res = g_lens.emplace(n, L);
+  return res.first->second;
+}
+
+void free_lens() {
+  std::lock_guard<std::mutex> lk(g_len_mu);
+  for (auto& kv : g_lens)
+    if (kv.second.tw) (void)hipFree(kv.second.tw);
+  g_lens.clear();
+}
+
+// Per-thread element budget of every stage (registers hold EPT complex values)
+// and the workgroup sizes the kernels are built for.
+static const int kEPT = 16;
+static const int kNTs[] = {64, 128, 256, 512, 1024};
+static const size_t kLdsMax = 160 * 1024;
+// data bytes per tile the tile-shape heuristics aim for (two tiles per CU)
+static const size_t kTileBytes = 64 * 1024;
+
+int max_len() { return 8192; }
+
+// elements one thread can own in every stage of this length
+static int len_cap_per_thread(const Len1D& L) {
+  int c = kEPT;
+  for (int s = 0; s < L.d.nst; s++) {
+    const int R = L.d.radix[s];
+    if (is_templated(R)) c = std::min(c, R * (kEPT / R));
+  }
+  return c;
+}
+
+static int pick_nt(long long elems, int cpt) {
+  for (int nt : kNTs)
+    if (elems <= (long long)nt * cpt) return nt;
+  throw Error{"transform tile exceeds the 1024-thread workgroup capacity", -1};
+}
+
+template <class K>
+static void allow_big_lds(K kernel) {
+  (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
+}
+
+// choose where twiddles live; returns LDS bytes
+static size_t lds_layout(size_t data_bytes, const Len1D& L, int* tw_lds, int* tw_off) {
+  size_t off = (data_bytes + 15) & ~(size_t)15;
+  size_t twb = (size_t)L.d.twcount * sizeof(Tw);
+  if (L.d.twcount > 0 && off + twb <= 96 * 1024) {
+    *tw_lds = 1;
+    *tw_off = (int)off;
+    return off + twb;
+  }
+  *tw_lds = 0;
+  *tw_off = 0;
+  return data_bytes;
+}
+
+constexpr int kEPTc = 16;
+
+// launch functors: one per kernel family, instantiated per workgroup size
+struct GoXr2c {
+  template <int NT>
+  static void go(const XArgs& a, unsigned g, size_t b, hipStream_t s) {
+    static bool once = (allow_big_lds(k_x_r2c<real_t, NT, kEPTc>), true);
+    (void)once;
+    k_x_r2c<real_t, NT, kEPTc><<<g, NT, b, s>>>(a);
+  }
+};
+struct GoXc2r {
+  template <int NT>
+  static void go(const XArgs& a, unsigned g, size_t b, hipStream_t s) {
+    static bool once = (allow_big_lds(k_x_c2r<real_t, NT, kEPTc>), true);
+    (void)once;
+    k_x_c2r<real_t, NT, kEPTc><<<g, NT, b, s>>>(a);
+  }
+};
+template <bool FWD>
+struct GoCol {
+  template <int NT>
+  static void go(const ColArgs& a, unsigned g, size_t b, hipStream_t s) {
+    static bool once = (allow_big_lds(k_col<real_t, FWD, NT, kEPTc>), true);
+    (void)once;
+    k_col<real_t, FWD, NT, kEPTc><<<g, NT, b, s>>>(a);
+  }
+};
+
+template <class F, class A>
+static void dispatch_nt(int nt, const A& a, unsigned g, size_t b, hipStream_t s) {
+  switch (nt) {
+    case 64: F::template go<64>(a, g, b, s); break;
+    case 128: F::template go<128>(a, g, b, s); break;
+    case 256: F::template go<256>(a, g, b, s); break;
+    case 512: F::template go<512>(a, g, b, s); break;
+    default: F::template go<1024>(a, g, b, s); break;
+  }
+}
+
+// row-pair count per X tile
+static long long x_pairs_per_tile(int nx, long long pairs) {
+  const long long tgt = (long long)(kTileBytes / sizeof(cplx_t));
+  long long B = std::max<long long>(1, tgt / nx);
+  // keep at least ~2048 workgroups in flight when the problem allows
+  B = std::min<long long>(B, std::max<long long>(1, pairs / 2048));
+  return B;
+}
+
+static void x_setup(XArgs& a, const Len1D& L, long long nrows, int nx, size_t* bytes, int* nt, unsigned* nwg) {
+  const long long pairs = (nrows + 1) / 2;
+  long long B = x_pairs_per_tile(nx, pairs);
+  const int cpt = len_cap_per_thread(L);
+  while (B > 1 && B * nx > 1024LL * cpt) B--;
+  a.B = (int)B;
+  a.L = L.d;
+  a.tw = L.tw;
+  const long long elems = B * nx;
+  *bytes = lds_layout((size_t)elems * sizeof(cplx_t), L, &a.tw_lds, &a.tw_off);
+  if (*bytes > kLdsMax) throw Error{"X transform does not fit in LDS", -1};
+  *nt = pick_nt(elems, cpt);
+  const long long g = (pairs + B - 1) / B;
+  if (g > 0x7fffffffLL) throw Error{"grid too large", -1};
+  *nwg = (unsigned)g;
+}
+
+void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_row, long long so_row, int nx,
+                  int nxo, hipStream_t st) {
+  if (nrows <= 0 || nxo <= 0) return;
+  const Len1D& L = get_len(nx);
+  XArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;
+  a.nxo = nxo; a.dnxo = make_fdiv(nxo);
+  size_t bytes; int nt; unsigned nwg;
+  x_setup(a, L, nrows, nx, &bytes, &nt, &nwg);
+  dispatch_nt<GoXr2c>(nt, a, nwg, bytes, st);
+  check_hip(hipGetLastError(), "k_x_r2c launch");
+}
+
+void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_row, long long so_row, int nx,
+                  int nxi, hipStream_t st) {
+  if (nrows <= 0) return;
+  const Len1D& L = get_len(nx);
+  XArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;
+  a.nxo = nxi; a.dnxo = make_fdiv(nxi > 0 ? nxi : 1);
+  size_t bytes; int nt; unsigned nwg;
+  x_setup(a, L, nrows, nx, &bytes, &nt, &nwg);
+  dispatch_nt<GoXc2r>(nt, a, nwg, bytes, st);
+  check_hip(hipGetLastError(), "k_x_c2r launch");
+}
+
+void launch_col(const ColSpec& c, hipStream_t st) {
+  if (c.n_inner <= 0 || c.n_outer <= 0 || c.nst <= 0) return;
+  ColArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = c.in; a.out = c.out;
+  a.si_in = c.si_in; a.sl_in = c.sl_in; a.so_in = c.so_in;
+  a.si_out = c.si_out; a.sl_out = c.sl_out; a.so_out = c.so_out;
+  a.n_inner = c.n_inner; a.n_outer = c.n_outer;
+  a.kind = c.kind;
+  a.nld = c.nld; a.ld_lo = c.ld_lo; a.ld_hi = c.ld_hi; a.ld_shift = c.ld_shift;
+  a.nst = c.nst; a.st_lo = c.st_lo; a.st_shift = c.st_shift;
+  int nfft;
+  const Len1D* L = nullptr;
+  if (c.kind == COL_COPY) nfft = c.nld;
+  else {
+    int m = c.n;
+    if (c.kind == COL_DCT1) {
+      if (c.n < 2) throw Error{"DCT-I needs at least 2 points", -1};
+      m = 2 * (c.n - 1);
+    } else if (c.kind == COL_DST1) m = 2 * (c.n + 1);
+    L = &get_len(m);
+    nfft = m;
+  }
+  const int rows = std::max(nfft, c.nld + (c.kind == COL_DST1 ? 2 : 0));
+  a.nrows_lds = rows;
+  const int cpt = L ? len_cap_per_thread(*L) : kEPT;
+  // tile width: one 128-byte segment per row where capacity allows
+  int TX = (int)(128 / sizeof(cplx_t));
+  const long long tgt = (long long)(kTileBytes / sizeof(cplx_t));
+  while (TX > 1 && (long long)TX * rows > std::max<long long>(tgt, rows)) TX /= 2;
+  while (TX > 1 && (long long)TX * rows > 1024LL * cpt) TX /= 2;
+  while (TX > 1 && TX / 2 >= c.n_inner) TX /= 2;
+  a.TX = TX;
+  a.dTX = make_fdiv(TX);
+  a.ld_lenfast = (c.sl_in == 1 && c.si_in != 1) ? 1 : 0;
+  a.st_lenfast = (c.sl_out == 1 && c.si_out != 1) ? 1 : 0;
+  a.dnld = make_fdiv(c.nld);
+  a.dnst = make_fdiv(c.nst);
+  a.tiles = (c.n_inner + TX - 1) / TX;
+  const long long nwg = (long long)a.tiles * c.n_outer;
+  if (nwg > 0x7fffffffLL) throw Error{"grid too large", -1};
+  const long long elems = (long long)TX * rows;
+  size_t bytes;
+  if (L) {
+    a.L = L->d;
+    a.tw = L->tw;
+    bytes = lds_layout((size_t)elems * sizeof(cplx_t), *L, &a.tw_lds, &a.tw_off);
+  } else {
+    a.L.n = nfft;
+    bytes = (size_t)elems * sizeof(cplx_t);
+  }
+  if (bytes > kLdsMax) throw Error{"column transform does not fit in LDS", -1};
+  const int nt = pick_nt(elems, cpt);
+  if (c.fwd) dispatch_nt<GoCol<true>>(nt, a, (unsigned)nwg, bytes, st);
+  else dispatch_nt<GoCol<false>>(nt, a, (unsigned)nwg, bytes, st);
+  check_hip(hipGetLastError(), "k_col launch");
+}
+
+void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long ks, double scale, double L,
+                  hipStream_t st) {
+  if (ncol <= 0) return;
+  const int nt = 256;
+  k_cheby<real_t><<<(unsigned)((ncol + nt - 1) / nt), nt, 0, st>>>(out, ncol, nzc, cs, ks, scale, L);
+  check_hip(hipGetLastError(), "k_cheby launch");
+}
+
+void launch_scale(cplx_t* p, long long n, double s, hipStream_t st) {
+  if (n <= 0) return;
+  k_scale<real_t><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, s);
+  check_hip(hipGetLastError(), "k_scale launch");
+}
+
+}  // namespace p3amd

@@ -1,0 +1,83 @@
+// engine.hpp -- host-side interface of the gfx950 FFT engine (kernels live in
+// engine.hip).  One build of the library instantiates one real type:
+// double by default, float when P3DFFT_SINGLE is defined (the reference's
+// SINGLE_PREC switch, build/module.F90:91-99).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fft_device.hpp"
+
+namespace p3amd {
+
+#ifdef P3DFFT_SINGLE
+typedef float real_t;
+#else
+typedef double real_t;
+#endif
+typedef cpx<real_t> cplx_t;
+typedef TwT<real_t>::type tw_t;
+
+// Library error: carries a message in the reference's wording.
+struct Error {
+  std::string msg;
+  int code;
+};
+
+void check_hip(hipError_t e, const char* what);
+
+// A 1D length with its stage schedule and device twiddle table.
+struct Len1D {
+  LenDesc d;
+  tw_t* tw = nullptr;  // device table: stage twiddles then (if needed) W_n^e
+  bool has_generic = false;
+};
+
+// Length cache (one per library instance).
+const Len1D& get_len(int n);
+void free_lens();
+
+// Largest transform length the engine runs in one workgroup.
+int max_len();
+
+// Row transforms along the contiguous axis (two rows per complex FFT).
+//   r2c: in real rows (stride si_row reals), out complex rows (stride so_row),
+//        keeping modes k < nxo.
+//   c2r: in complex rows with nxi valid modes (the rest read as zero),
+//        out real rows.
+void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_row, long long so_row,
+                  int nx, int nxo, hipStream_t st);
+void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_row, long long so_row,
+                  int nx, int nxi, hipStream_t st);
+
+// Column transform along a strided axis of a 3D view [outer][len][inner].
+enum ColKind { COL_C2C = 0, COL_DCT1 = 1, COL_DST1 = 2, COL_COPY = 3 };
+struct ColSpec {
+  const cplx_t* in;
+  cplx_t* out;
+  long long si_in, sl_in, so_in;     // input strides (complex elements): inner, length, outer
+  long long si_out, sl_out, so_out;  // output strides
+  int n_inner, n_outer;
+  int n;          // logical transform length (rows of the column)
+  int kind;       // ColKind
+  bool fwd;       // forward (e^{-i}) or backward
+  int nld;        // rows filled from the input (<= n)
+  int ld_lo, ld_hi, ld_shift;  // LDS row e <- input row e (e<ld_lo), e-ld_shift (e>=ld_hi), else 0
+  int nst;        // rows written to the output
+  int st_lo, st_shift;         // output row e <- LDS row e (e<st_lo) or e+st_shift
+};
+void launch_col(const ColSpec& c, hipStream_t st);
+
+// Chebyshev post-processing of a 'ffc' spectrum (build/ftran.F90:408-451):
+// scale, then the derivative recurrence along z.  Element (col c, z k) at
+// out[c*cs + k*ks].
+void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long ks, double scale,
+                  double L, hipStream_t st);
+
+// out[i] *= s over n complex elements (used by callers that normalise)
+void launch_scale(cplx_t* p, long long n, double s, hipStream_t st);
+
+}  // namespace p3amd

@@ -1,0 +1,140 @@
+! p3dfft.f90 -- Fortran module `p3dfft` of libp3dfft_amd.
+!
+! Same module name, kinds and procedure names as the reference's module
+! (build/module.F90:81-186) for the ftran_r2c / btran_c2r path, with the
+! optional arguments of p3dfft_setup (build/setup.F90:107-120).  Every call
+! forwards through iso_c_binding to the C-ABI of libp3dfft_amd.so (or, built
+! with -DSINGLE_PREC, libp3dfft_amd_sp.so).  Arrays are host arrays; the
+! library copies them to and from HBM inside each call.
+module p3dfft
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+#ifndef SINGLE_PREC
+  integer, parameter, public :: p3dfft_type = kind(1.0d0)
+#else
+  integer, parameter, public :: p3dfft_type = kind(1.0)
+#endif
+  integer, parameter, public :: r8 = kind(1.0d0)
+  integer, parameter, public :: i8 = selected_int_kind(18)
+  ! stage timers (build/module.F90:106), refreshed after every transform call
+  real(r8), save, public :: timers(12) = 0.0_r8
+
+  public :: p3dfft_setup, p3dfft_get_dims, p3dfft_ftran_r2c, p3dfft_btran_c2r, &
+            p3dfft_cheby, p3dfft_clean, get_timers, set_timers
+
+  interface
+    subroutine c_p3dfft_setup(dims, nx, ny, nz, comm, nxc, nyc, nzc, ow, memsize) &
+        bind(C, name='p3dfft_setup')
+      import :: c_int
+      integer(c_int) :: dims(2), nx, ny, nz, comm, nxc, nyc, nzc, ow, memsize(3)
+    end subroutine
+    subroutine c_p3dfft_get_dims(st, en, sz, conf) bind(C, name='p3dfft_get_dims')
+      import :: c_int
+      integer(c_int) :: st(3), en(3), sz(3), conf
+    end subroutine
+    subroutine c_p3dfft_ftran_r2c(a, b, op) bind(C, name='p3dfft_ftran_r2c')
+      import :: c_ptr, c_char
+      type(c_ptr), value :: a, b
+      character(kind=c_char) :: op(*)
+    end subroutine
+    subroutine c_p3dfft_btran_c2r(a, b, op) bind(C, name='p3dfft_btran_c2r')
+      import :: c_ptr, c_char
+      type(c_ptr), value :: a, b
+      character(kind=c_char) :: op(*)
+    end subroutine
+    subroutine c_p3dfft_cheby(a, b, lz) bind(C, name='p3dfft_cheby')
+      import :: c_ptr
+      type(c_ptr), value :: a, b, lz
+    end subroutine
+    subroutine c_p3dfft_clean() bind(C, name='p3dfft_clean')
+    end subroutine
+    subroutine c_get_timers(t) bind(C, name='get_timers')
+      import :: c_double
+      real(c_double) :: t(12)
+    end subroutine
+    subroutine c_set_timers() bind(C, name='set_timers')
+    end subroutine
+  end interface
+
+contains
+
+  ! build/setup.F90:107-120: cut sizes default to the grid, overwrite to .true.
+  subroutine p3dfft_setup(dims, nx, ny, nz, mpi_comm_in, nxcut, nycut, nzcut, overwrite, memsize)
+    integer, intent(in) :: dims(2), nx, ny, nz, mpi_comm_in
+    integer, intent(in), optional :: nxcut, nycut, nzcut
+    logical, intent(in), optional :: overwrite
+    integer, intent(out), optional :: memsize(3)
+    integer(c_int) :: d(2), cnx, cny, cnz, comm, cx, cy, cz, ow, ms(3)
+    d = dims
+    cnx = nx; cny = ny; cnz = nz; comm = mpi_comm_in
+    cx = nx; cy = ny; cz = nz; ow = 1
+    if (present(nxcut)) cx = nxcut
+    if (present(nycut)) cy = nycut
+    if (present(nzcut)) cz = nzcut
+    if (present(overwrite)) then
+      if (.not. overwrite) ow = 0
+    end if
+    call c_p3dfft_setup(d, cnx, cny, cnz, comm, cx, cy, cz, ow, ms)
+    if (present(memsize)) memsize = ms
+    timers = 0.0_r8
+  end subroutine
+
+  ! build/module.F90:225-273
+  subroutine p3dfft_get_dims(istart, iend, isize, conf)
+    integer, intent(out) :: istart(3), iend(3), isize(3)
+    integer, intent(in) :: conf
+    integer(c_int) :: s(3), e(3), z(3), c
+    c = conf
+    call c_p3dfft_get_dims(s, e, z, c)
+    istart = s; iend = e; isize = z
+  end subroutine
+
+  ! build/ftran.F90:489 (arrays of any rank by sequence association)
+  subroutine p3dfft_ftran_r2c(XgYZ, XYZg, op)
+    real(p3dfft_type), target :: XgYZ(*)
+    complex(p3dfft_type), target :: XYZg(*)
+    character(len=*), intent(in) :: op
+    call c_p3dfft_ftran_r2c(c_loc(XgYZ(1)), c_loc(XYZg(1)), op(1:3)//c_null_char)
+    call c_get_timers(timers)
+  end subroutine
+
+  ! build/btran.F90:396
+  subroutine p3dfft_btran_c2r(XYZg, XgYZ, op)
+    complex(p3dfft_type), target :: XYZg(*)
+    real(p3dfft_type), target :: XgYZ(*)
+    character(len=*), intent(in) :: op
+    call c_p3dfft_btran_c2r(c_loc(XYZg(1)), c_loc(XgYZ(1)), op(1:3)//c_null_char)
+    call c_get_timers(timers)
+  end subroutine
+
+  ! build/ftran.F90:383
+  subroutine p3dfft_cheby(in, out, Lz)
+    real(p3dfft_type), target :: in(*)
+    complex(p3dfft_type), target :: out(*)
+    real(p3dfft_type), intent(in) :: Lz
+    real(p3dfft_type), target :: lzc
+    lzc = Lz
+    call c_p3dfft_cheby(c_loc(in(1)), c_loc(out(1)), c_loc(lzc))
+    call c_get_timers(timers)
+  end subroutine
+
+  ! build/module.F90:309
+  subroutine p3dfft_clean()
+    call c_p3dfft_clean()
+  end subroutine
+
+  ! build/module.F90:730-744
+  subroutine get_timers(timer)
+    real(r8), intent(out) :: timer(12)
+    call c_get_timers(timer)
+    timers = timer
+  end subroutine
+
+  subroutine set_timers()
+    call c_set_timers()
+    timers = 0.0_r8
+  end subroutine
+
+end module p3dfft

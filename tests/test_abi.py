@@ -1,0 +1,92 @@
+"""CPU checks of the drop-in boundary: the C-ABI libraries load, export every
+entry point include/p3dfft.h declares, and the host-only paths behave as the
+reference does (no GPU calls are made here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import p3dfft_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "p3dfft.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set()
+    for m in re.finditer(r"^(?!static)[A-Za-z_][\w \t\*]*?\b(\w+)\s*\([^;{]*\)\s*;", src, flags=re.M):
+        names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_reference_entry_points():
+    names = declared_symbols()
+    for ref in ("p3dfft_setup", "p3dfft_get_dims", "p3dfft_ftran_r2c", "p3dfft_btran_c2r", "p3dfft_cheby",
+                "p3dfft_clean", "get_timers", "set_timers"):
+        assert ref in names
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_library_exports_every_declared_symbol(precision):
+    L = p3dfft_amd.load(precision)
+    for name in declared_symbols():
+        assert hasattr(L, name), "%s missing from %s" % (name, p3dfft_amd.lib_path(precision))
+    assert L.p3dfft_amd_precision() == precision
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_dynamic_symbol_table(precision):
+    out = subprocess.run(["nm", "-D", "--defined-only", p3dfft_amd.lib_path(precision)],
+                         stdout=subprocess.PIPE, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for name in declared_symbols():
+        assert name in exported
+
+
+def test_call_before_setup_prints_and_returns():
+    """build/module.F90:237-239: get_dims before setup prints an error and returns."""
+    L = p3dfft_amd.load(8)
+    L.p3dfft_amd_set_error_mode(1)
+    L.p3dfft_amd_clear_error()
+    s = (ctypes.c_int * 3)(7, 7, 7)
+    e = (ctypes.c_int * 3)(7, 7, 7)
+    z = (ctypes.c_int * 3)(7, 7, 7)
+    c = ctypes.c_int(1)
+    L.p3dfft_get_dims(s, e, z, ctypes.byref(c))
+    assert "call setup before other routines" in L.p3dfft_amd_last_error().decode()
+    assert list(s) == [7, 7, 7]
+    L.p3dfft_amd_clear_error()
+
+
+def test_invalid_dims_rejected_without_gpu():
+    """build/setup.F90:125-128 and 181-184: invalid dimensions and processor
+    geometry are rejected before any device work."""
+    P = p3dfft_amd.P3DFFT(8)
+    with pytest.raises(p3dfft_amd.P3DFFTError, match="Invalid dimensions"):
+        P.setup((1, 1), 0, 8, 8)
+    with pytest.raises(p3dfft_amd.P3DFFTError, match="Invalid processor geometry"):
+        P.setup((2, 1), 8, 8, 8)
+    with pytest.raises(p3dfft_amd.P3DFFTError, match="Invalid cut dimensions"):
+        P.setup((1, 1), 8, 8, 8, nxc=9)
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "p3dfft_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h", ".f90")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "from oracle" not in text, f
+                assert "p3dfft_oracle" not in text, f
+
+
+def test_fortran_module_built():
+    if not os.path.exists("/opt/rocm/bin/amdflang"):
+        pytest.skip("amdflang not present")
+    lib = os.path.join(ROOT, "p3dfft_amd", "lib")
+    assert os.path.exists(os.path.join(lib, "mod", "p3dfft.mod"))
+    assert os.path.exists(os.path.join(lib, "libp3dfft_amd_f.so"))
+    assert os.path.exists(os.path.join(lib, "mod_sp", "p3dfft.mod"))

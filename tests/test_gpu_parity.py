@@ -1,0 +1,316 @@
+"""GPU parity of the HIP path against the FP64 CPU oracle, through the C-ABI.
+
+Tolerances (relative L2 unless stated):
+  FP64 library vs FP64 oracle:  1e-12
+  FP32 library (compensated hi+lo twiddles) vs FP64 oracle:  2e-6
+  round trips additionally meet the reference drivers' absolute bound
+  prec * N / 4 with prec = 1e-14 (FP64) or 1e-5 (FP32)
+  (sample/FORTRAN/driver_sine.F90:304-314).
+"""
+import contextlib
+
+import numpy as np
+import pytest
+
+import p3dfft_amd
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {8: 1e-12, 4: 2e-6}
+PREC = {8: 1e-14, 4: 1e-5}
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    assert p3dfft_amd.device_count() > 0, "no GPU visible to the HIP runtime"
+
+
+@contextlib.contextmanager
+def lib_setup(precision, nx, ny, nz, nxc=None, nyc=None, nzc=None, ow=True, stride1=False):
+    P = p3dfft_amd.P3DFFT(precision)
+    P.setup((1, 1), nx, ny, nz, nxc, nyc, nzc, overwrite=ow, stride1=stride1)
+    try:
+        yield P
+    finally:
+        P.clean()
+
+
+def oplan(nx, ny, nz, nxc=None, nyc=None, nzc=None, ow=True, stride1=False):
+    return O.Plan(nx, ny, nz, nxc or 0, nyc or 0, nzc or 0, ow=ow, stride1=stride1)
+
+
+# ---------------------------------------------------------------- C1: 64^3 FP64
+def test_c1_64cube_fp64_against_oracle():
+    nx = ny = nz = 64
+    a = np.random.default_rng(1).random((nz, ny, nx))
+    ref = oplan(nx, ny, nz).ftran_r2c(a)
+    with lib_setup(8, nx, ny, nz) as P:
+        assert P.memsize == (64, 64, 66)
+        assert P.get_dims(2) == ((1, 1, 1), (33, 64, 64), (33, 64, 64))
+        s = P.ftran_r2c(a, "fft")
+        assert rel(s, ref) < 1e-12
+        back = P.btran_c2r(s / (nx * ny * nz), "tff")
+    assert np.abs(back - a).max() < PREC[8] * nx * ny * nz * 0.25
+    assert rel(back, a) < 1e-13
+
+
+# ---------------------------------------------------------------- every Z option
+SHAPES = [(8, 6, 10), (16, 16, 16), (12, 9, 5), (7, 11, 13), (10, 14, 6), (6, 4, 17), (32, 24, 20)]
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_forward_ops(precision, shape):
+    nx, ny, nz = shape
+    a = np.random.default_rng(sum(shape)).standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz)
+    with lib_setup(precision, nx, ny, nz) as P:
+        for op in ("fft", "fff", "ffc", "ffs", "ffn", "ff0"):
+            if op == "ffc" and nz < 2:
+                continue
+            assert rel(P.ftran_r2c(a, op), o.ftran_r2c(a, op)) < TOL[precision], op
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("ow", [True, False])
+def test_backward_ops(precision, shape, ow):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(sum(shape) + 1)
+    s = rng.standard_normal((nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((nz, ny, nx // 2 + 1))
+    o = oplan(nx, ny, nz, ow=ow)
+    with lib_setup(precision, nx, ny, nz, ow=ow) as P:
+        for op in ("tff", "fff", "cff", "sff", "nff", "0ff"):
+            assert rel(P.btran_c2r(s, op), o.btran_c2r(s, op)[0]) < TOL[precision], op
+
+
+# ---------------------------------------------------------------- transform lengths
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape", [(14, 26, 38), (22, 39, 17), (96, 160, 48), (30, 20, 44), (19, 23, 29),
+                                   (128, 40, 56), (34, 2, 3), (2, 3, 2)])
+def test_radix_coverage_round_trip(precision, shape):
+    """radix 2/3/4/5/7/8/11/13, generic primes (17, 19, 23, 29) and the
+    reference's uneven grid 14 26 38 (extra/makejob.py:142-157)."""
+    nx, ny, nz = shape
+    N = nx * ny * nz
+    a = np.random.default_rng(N).random((nz, ny, nx))
+    o = oplan(nx, ny, nz)
+    ref = o.ftran_r2c(a)
+    with lib_setup(precision, nx, ny, nz) as P:
+        s = P.ftran_r2c(a)
+        assert rel(s, ref) < TOL[precision]
+        back = P.btran_c2r(s, "tff") / N
+    assert np.abs(back - a).max() < PREC[precision] * N * 0.25
+
+
+# ---------------------------------------------------------------- pruning, layout, OW
+CUTS = [(12, 12, 10), (8, 12, 10), (6, 8, 6), (10, 9, 7), (12, 7, 8), (7, 12, 9)]
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("cut", CUTS)
+@pytest.mark.parametrize("stride1", [False, True])
+def test_pruned_forward(precision, cut, stride1):
+    nx, ny, nz = 12, 12, 10
+    a = np.random.default_rng(3).standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz, *cut, stride1=stride1)
+    with lib_setup(precision, nx, ny, nz, *cut, stride1=stride1) as P:
+        assert P.get_dims(2) == o.get_dims(2)
+        assert P.get_dims(3) == o.get_dims(3)
+        for op in ("fft", "ffc", "ffs", "ffn"):
+            assert rel(P.ftran_r2c(a, op), o.ftran_r2c(a, op)) < TOL[precision], op
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("cut", CUTS)
+@pytest.mark.parametrize("stride1", [False, True])
+@pytest.mark.parametrize("ow", [True, False])
+def test_pruned_backward(precision, cut, stride1, ow):
+    nx, ny, nz = 12, 12, 10
+    o = oplan(nx, ny, nz, *cut, ow=ow, stride1=stride1)
+    rng = np.random.default_rng(4)
+    shp = o.spec_shape()
+    s = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+    with lib_setup(precision, nx, ny, nz, *cut, ow=ow, stride1=stride1) as P:
+        for op in ("tff", "cff", "sff", "nff"):
+            assert rel(P.btran_c2r(s, op), o.btran_c2r(s, op)[0]) < TOL[precision], op
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("ow", [True, False])
+def test_device_api_overwrite_and_in_place(precision, ow):
+    """btran with overwrite and nz==nzc transforms the caller's (device) input
+    in place along Z (build/btran.F90:525-550), as the oracle restates;
+    without overwrite the input is preserved.  IN==OUT works both ways."""
+    nx, ny, nz = 16, 12, 10
+    rng = np.random.default_rng(6)
+    a = rng.standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz, ow=ow)
+    ref_s = o.ftran_r2c(a)
+    ref_b, ref_in_after = o.btran_c2r(ref_s, "tff", keep_input=False)
+    with lib_setup(precision, nx, ny, nz, ow=ow) as P:
+        nb = max(P.real_bytes(), P.spec_bytes())
+        buf = P.alloc(nb)
+        buf.upload(a.astype(P.rdtype))
+        P.ftran_r2c_dev(buf.ptr, buf.ptr, "fft")          # in place
+        s = buf.download(P.spec_shape(), P.cdtype)
+        assert rel(s, ref_s) < TOL[precision]
+        out = P.alloc(nb)
+        P.btran_c2r_dev(buf.ptr, out.ptr, "tff")
+        P.synchronize()
+        assert rel(out.download(P.real_shape(), P.rdtype), ref_b) < TOL[precision]
+        after = buf.download(P.spec_shape(), P.cdtype)
+        assert rel(after, ref_in_after) < TOL[precision]
+        if not ow:
+            assert np.array_equal(after, s)
+        buf.free(); out.free()
+
+
+# ---------------------------------------------------------------- Chebyshev
+def cheby_points(nz, Lz):
+    return np.cos(np.pi * np.arange(nz) / (nz - 1)) * 2.0 / Lz
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_driver_cheby(precision):
+    """sample/FORTRAN/driver_cheby.F90:218-285 restated: cheby of sin(z) on
+    Gauss-Lobatto points, end coefficients doubled, halved, btran 'cff' = cos(z)."""
+    nx, ny, nz, Lz = 32, 32, 33, 2.0
+    z = cheby_points(nz, Lz)
+    a = np.broadcast_to(np.sin(z)[:, None, None], (nz, ny, nx)).copy()
+    o = oplan(nx, ny, nz)
+    with lib_setup(precision, nx, ny, nz) as P:
+        c = P.cheby(a, Lz)
+        assert rel(c, o.cheby(a, Lz)) < TOL[precision]
+        c = c.astype(np.complex128)
+        c[0] *= 2.0
+        c[nz - 1] *= 2.0
+        c *= 0.5
+        d = P.btran_c2r(c, "cff")
+    assert np.abs(d - np.cos(z)[:, None, None]).max() < PREC[precision] * nx * ny * nz * 0.25
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("stride1", [False, True])
+def test_cheby_random_polynomial(precision, stride1):
+    """The Chebyshev configuration at reduced size: trig modes in x,y times a
+    random degree-<=64 Chebyshev polynomial in z (coefficients N(0,1)/(1+n)^2)."""
+    nx, ny, nz, Lz = 64, 64, 129, 2.0
+    rng = np.random.default_rng(44)
+    c = rng.standard_normal(65) / (1 + np.arange(65)) ** 2
+    zz = cheby_points(nz, Lz)
+    g = np.polynomial.chebyshev.chebval(zz, c)
+    xs = np.arange(nx) / nx
+    ys = np.arange(ny) / ny
+    xy = sum(np.cos(2 * np.pi * (rng.integers(1, 8) * xs[None, :] + rng.integers(1, 8) * ys[:, None])
+                    + rng.random() * 2 * np.pi) for _ in range(4))
+    a = g[:, None, None] * xy[None]
+    o = oplan(nx, ny, nz, stride1=stride1)
+    with lib_setup(precision, nx, ny, nz, stride1=stride1) as P:
+        assert rel(P.cheby(a, Lz), o.cheby(a, Lz)) < TOL[precision]
+
+
+# ---------------------------------------------------------------- errors, timers
+def test_error_paths():
+    P = p3dfft_amd.P3DFFT(8)
+    P.setup((1, 1), 8, 8, 8)
+    try:
+        with pytest.raises(p3dfft_amd.P3DFFTError, match="already initialized"):
+            P.setup((1, 1), 8, 8, 8)
+        with pytest.raises(p3dfft_amd.P3DFFTError, match="Unknown transform type"):
+            P.ftran_r2c(np.zeros((8, 8, 8)), "ffx")
+        with pytest.raises(p3dfft_amd.P3DFFTError, match="Unknown transform type"):
+            P.btran_c2r(np.zeros((8, 8, 5), np.complex128), "xff")
+    finally:
+        P.clean()
+    # a new setup is allowed after clean (module.F90:424 resets mpi_set)
+    P.setup((1, 1), 4, 4, 4)
+    P.clean()
+
+
+def test_timers_accumulate():
+    nx = ny = nz = 32
+    a = np.random.default_rng(0).random((nz, ny, nx))
+    with lib_setup(8, nx, ny, nz) as P:
+        P.set_timers()
+        assert P.get_timers() == [0.0] * 12
+        s = P.ftran_r2c(a)
+        P.btran_c2r(s)
+        t = P.get_timers()
+        for slot in (5, 7, 8, 9, 10, 12):
+            assert t[slot - 1] > 0.0, slot
+        for slot in (1, 2, 3, 4, 6, 11):
+            assert t[slot - 1] == 0.0
+        P.set_timers()
+        assert P.get_timers() == [0.0] * 12
+
+
+# ---------------------------------------------------------------- C2 / C3 at full size
+def modes_field(n, seed, dtype):
+    """sum of 8 random-phase Fourier modes (integer wavenumbers in [1,32]) plus
+    N(0, 1e-3^2) noise, evaluated in FP64."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, 33, size=(8, 3))
+    ph = rng.random(8) * 2 * np.pi
+    x = np.arange(n) / n
+    f = np.zeros((n, n, n))
+    for m in range(8):
+        f += np.cos(2 * np.pi * (k[m, 0] * x[None, None, :] + k[m, 1] * x[None, :, None]
+                                 + k[m, 2] * x[:, None, None]) + ph[m])
+    f += rng.normal(0.0, 1e-3, size=f.shape)
+    return f.astype(dtype), k, ph
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_c2_512cube_modes_and_round_trip(precision):
+    n = 512
+    N = n ** 3
+    f, k, ph = modes_field(n, 2026, np.float64 if precision == 8 else np.float32)
+    with lib_setup(precision, n, n, n) as P:
+        buf = P.alloc(P.spec_bytes())
+        buf.upload(f)
+        spec_d = P.alloc(P.spec_bytes())
+        P.ftran_r2c_dev(buf.ptr, spec_d.ptr, "fft")
+        s = spec_d.download(P.spec_shape(), P.cdtype)
+        # each mode cos(2 pi k.x + ph) contributes N/2 e^{i ph} at +k (kx>0)
+        for m in range(8):
+            v = s[k[m, 2], k[m, 1], k[m, 0]]
+            assert abs(v - N / 2 * np.exp(1j * ph[m])) < 1e-3 * N, m
+        P.btran_c2r_dev(spec_d.ptr, buf.ptr, "tff")
+        back = buf.download(P.real_shape(), P.rdtype).astype(np.float64) / N
+        buf.free(); spec_d.free()
+    err = back - f.astype(np.float64)
+    assert np.abs(err).max() < PREC[precision] * N * 0.25
+    assert np.linalg.norm(err) / np.linalg.norm(f.astype(np.float64)) < (1e-14 if precision == 8 else 2e-6)
+
+
+def test_c2_512cube_fp64_spectrum_against_oracle():
+    n = 512
+    f, _, _ = modes_field(n, 7, np.float64)
+    ref = oplan(n, n, n).ftran_r2c(f)
+    with lib_setup(8, n, n, n) as P:
+        s = P.ftran_r2c(f)
+    assert rel(s, ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [384, 512])
+def test_c3_precision_fp32_against_fp64(n):
+    """FP32 bulk with compensated twiddles vs the FP64 path on i.i.d. N(0,1)."""
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n, n))
+    with lib_setup(8, n, n, n) as P:
+        s64 = P.ftran_r2c(a)
+        b64 = P.btran_c2r(s64) / n ** 3
+    with lib_setup(4, n, n, n) as P:
+        s32 = P.ftran_r2c(a.astype(np.float32))
+        b32 = P.btran_c2r(s32.copy()) / n ** 3
+    assert rel(s32, s64) < 2e-6
+    assert rel(b32, a) < 2e-6
+    assert rel(b64, a) < 1e-14
