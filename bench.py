@@ -1,0 +1,205 @@
+"""Benchmark of the forward+backward 3D FFT path on one MI355X.
+
+One step = p3dfft_ftran_r2c ('fft') + p3dfft_btran_c2r ('tff') of a 512^3 real
+field through the device-pointer C-ABI entry points, with the field, the
+spectrum and the library's work buffers resident in HBM (no host traffic in
+the timed region).  The headline is the FP32 build (FP32 bulk, compensated
+hi+lo FP32 twiddles generated in FP64); the plain FP64 build is timed beside
+it on the same field.
+
+Prints ONE JSON line.  Fields beyond the driver's contract:
+  roofline       dominant kernel: algorithmic bytes per launch / HIP-event
+                 duration per launch, against the 8 TB/s HBM peak
+  path           whole fwd+bwd against the compulsory-bytes roofline
+                 2B = 2 (N_r s + N_c 2s) and the measured copy-kernel peak
+  fp64_*         the plain FP64 GPU path and the FP32 speed-up over it
+  rel_l2_*       FP32 spectrum / round trip vs the FP64 path
+  cpu_baseline   the FP64 CPU oracle (a restatement of the reference's CPU
+                 algorithm) on a bounded sample, host cores of this box
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import p3dfft_amd  # noqa: E402
+
+HBM_PEAK = 8.0e12
+METRIC = "fwd+bwd 3D FFT time & % HBM roofline at 512^3; batched/served solves/s, p99"
+SLOT_NAMES = {5: "x_r2c (k_x_r2c)", 7: "y_fwd (k_col)", 8: "z_fwd (k_col)", 9: "z_bwd (k_col)",
+              10: "y_bwd (k_col)", 12: "x_c2r (k_x_c2r)"}
+
+
+def modes_field(n, seed):
+    """sum of 8 random-phase Fourier modes (wavenumbers in [1,32]) + N(0,1e-3^2)."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, 33, size=(8, 3))
+    ph = rng.random(8) * 2 * np.pi
+    x = np.arange(n) / n
+    f = np.zeros((n, n, n))
+    for m in range(8):
+        f += np.cos(2 * np.pi * (k[m, 0] * x[None, None, :] + k[m, 1] * x[None, :, None]
+                                 + k[m, 2] * x[:, None, None]) + ph[m])
+    f += rng.normal(0.0, 1e-3, size=f.shape)
+    return f
+
+
+def slot_bytes(slot, n, s):
+    """algorithmic bytes per launch of the stage kernel in a timer slot"""
+    nr = n ** 3
+    nc = (n // 2 + 1) * n * n
+    if slot in (5, 12):
+        return nr * s + nc * 2 * s
+    return 2 * nc * 2 * s
+
+
+def run_precision(precision, n, f, steps, warmup):
+    P = p3dfft_amd.P3DFFT(precision)
+    P.setup((1, 1), n, n, n)
+    lib = P.lib
+    try:
+        fin = P.alloc(P.real_bytes())
+        spec = P.alloc(P.spec_bytes())
+        out = P.alloc(P.real_bytes())
+        fin.upload(f.astype(P.rdtype))
+        for _ in range(warmup):
+            P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft")
+            P.btran_c2r_dev(spec.ptr, out.ptr, "tff")
+        P.synchronize()
+        P.set_timers()
+        e0 = lib.p3dfft_amd_event_create()
+        e1 = lib.p3dfft_amd_event_create()
+        P.synchronize()
+        t0 = time.perf_counter()
+        lib.p3dfft_amd_event_record(e0, None)
+        for _ in range(steps):
+            P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft")
+            P.btran_c2r_dev(spec.ptr, out.ptr, "tff")
+        lib.p3dfft_amd_event_record(e1, None)
+        P.synchronize()
+        wall = time.perf_counter() - t0
+        ev_ms = lib.p3dfft_amd_event_elapsed_ms(e0, e1)
+        lib.p3dfft_amd_event_destroy(e0)
+        lib.p3dfft_amd_event_destroy(e1)
+        timers = P.get_timers()
+        # one more forward for the accuracy comparison
+        P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft")
+        s = spec.download(P.spec_shape(), P.cdtype)
+        P.btran_c2r_dev(spec.ptr, out.ptr, "tff")
+        back = out.download(P.real_shape(), P.rdtype)
+        fin.free(); spec.free(); out.free()
+    finally:
+        P.clean()
+    return {"ms_per_step": ev_ms / steps, "wall_ms_per_step": wall * 1e3 / steps, "timers": timers,
+            "spec": s, "back": back}
+
+
+def copy_peak(lib, nbytes=2 << 30, reps=10):
+    a = p3dfft_amd.DeviceBuffer(lib, nbytes)
+    b = p3dfft_amd.DeviceBuffer(lib, nbytes)
+    lib.p3dfft_amd_memset(a.ptr, 0, nbytes)
+    lib.p3dfft_amd_stream_copy(a.ptr, b.ptr, nbytes, None)
+    e0 = lib.p3dfft_amd_event_create()
+    e1 = lib.p3dfft_amd_event_create()
+    lib.p3dfft_amd_synchronize()
+    lib.p3dfft_amd_event_record(e0, None)
+    for _ in range(reps):
+        lib.p3dfft_amd_stream_copy(a.ptr, b.ptr, nbytes, None)
+    lib.p3dfft_amd_event_record(e1, None)
+    ms = lib.p3dfft_amd_event_elapsed_ms(e0, e1)
+    lib.p3dfft_amd_event_destroy(e0)
+    lib.p3dfft_amd_event_destroy(e1)
+    a.free(); b.free()
+    return 2.0 * nbytes * reps / (ms * 1e-3)
+
+
+def cpu_baseline(n, f, threads):
+    """FP64 CPU oracle (restatement of the reference CPU algorithm, OpenMP over
+    1D lines) on one fwd+bwd of the same field."""
+    from oracle import oracle as O
+    O.set_threads(threads)
+    p = O.Plan(n, n, n)
+    t0 = time.perf_counter()
+    s = p.ftran_r2c(f, "fft")
+    p.btran_c2r(s, "tff", keep_input=False)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "fwd+bwd/s", "cores": O.get_threads(), "kind": "port",
+            "sample": "one fwd+bwd (fft/tff) of the same %d^3 field in FP64 by the C oracle, %.2f s"
+                      % (n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the multi-GPU exchange is a design, "
+              "not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    f = modes_field(n, 2026)
+    r32 = run_precision(4, n, f, args.steps, args.warmup)
+    r64 = None if args.no_fp64 else run_precision(8, n, f, args.steps, args.warmup)
+    lib = p3dfft_amd.load(4)
+    cpk = copy_peak(lib)
+
+    s = 4
+    nr, nc = n ** 3, (n // 2 + 1) * n * n
+    B = nr * s + nc * 2 * s
+    t = r32["ms_per_step"] * 1e-3
+    kern = {}
+    for slot, name in SLOT_NAMES.items():
+        avg = r32["timers"][slot - 1] / args.steps
+        kern[name] = {"avg_ms": avg * 1e3, "gbps": slot_bytes(slot, n, s) / avg / 1e9 if avg > 0 else None}
+    dom_slot = max(SLOT_NAMES, key=lambda k: r32["timers"][k - 1])
+    dom_avg = r32["timers"][dom_slot - 1] / args.steps
+    achieved = slot_bytes(dom_slot, n, s) / dom_avg
+    line = {
+        "metric": METRIC,
+        "value": 1.0 / t,
+        "unit": "fwd+bwd 512^3 transforms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 8 random-phase Fourier modes + N(0,1e-3) noise, seed 2026",
+        "config": {"workload": "%d^3 r2c 'fft' + c2r 'tff', FP32 bulk, compensated twiddles, 1x1 grid, "
+                               "fields resident in HBM" % n, "grid": [n, n, n], "op": ["fft", "tff"]},
+        "roofline": {"bound": "hbm", "kernel": SLOT_NAMES[dom_slot], "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "bytes_per_launch": slot_bytes(dom_slot, n, s), "avg_launch_ms": dom_avg * 1e3},
+        "path": {"compulsory_bytes": 2 * B, "frac_of_8TBps": 2 * B / t / HBM_PEAK,
+                 "copy_kernel_GBps": cpk / 1e9, "frac_of_copy_peak": 2 * B / t / cpk,
+                 "gflops_5NlogN": 2 * 5 * nr * np.log2(nr) / t / 1e9, "wall_ms_per_step": r32["wall_ms_per_step"]},
+        "kernels": kern,
+    }
+    if r64 is not None:
+        ref = r64["spec"].astype(np.complex128)
+        line["fp64_ms_per_step"] = r64["ms_per_step"]
+        line["speedup_fp32_vs_fp64"] = r64["ms_per_step"] / r32["ms_per_step"]
+        line["rel_l2_spectrum_fp32_vs_fp64"] = float(np.linalg.norm(r32["spec"] - ref) / np.linalg.norm(ref))
+        b64 = r64["back"].astype(np.float64)
+        line["rel_l2_roundtrip_fp32_vs_fp64"] = float(np.linalg.norm(r32["back"] - b64) / np.linalg.norm(b64))
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n, f, args.cpu_threads)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -111,6 +111,8 @@ int p3dfft_amd_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int p3dfft_amd_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int p3dfft_amd_memset(void *dst, int value, size_t bytes);
 int p3dfft_amd_synchronize(void);
+/* streaming copy kernel (bytes a multiple of 16): HBM bandwidth reference */
+int p3dfft_amd_stream_copy(const void *in, void *out, size_t bytes, void *stream);
 int p3dfft_amd_device_count(void);
 void *p3dfft_amd_event_create(void);
 int p3dfft_amd_event_record(void *event, void *stream);

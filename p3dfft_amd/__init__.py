@@ -77,6 +77,8 @@ def load(precision=8):
         getattr(L, n).restype = ctypes.c_int
     L.p3dfft_amd_memset.argtypes = [_vp, ctypes.c_int, ctypes.c_size_t]
     L.p3dfft_amd_synchronize.restype = ctypes.c_int
+    L.p3dfft_amd_stream_copy.argtypes = [_vp, _vp, ctypes.c_size_t, _vp]
+    L.p3dfft_amd_stream_copy.restype = ctypes.c_int
     L.p3dfft_amd_device_count.restype = ctypes.c_int
     L.p3dfft_amd_event_create.restype = _vp
     L.p3dfft_amd_event_record.argtypes = [_vp, _vp]

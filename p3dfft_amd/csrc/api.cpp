@@ -548,6 +548,19 @@ int p3dfft_amd_device_count(void) {
   return n;
 }
 
+// Streaming device copy of `bytes` (a multiple of 16) with the library's copy
+// kernel: the achievable-HBM-bandwidth reference for roofline figures.
+int p3dfft_amd_stream_copy(const void* in, void* out, size_t bytes, void* stream) {
+  if (bytes % 16) return -1;
+  try {
+    launch_copy16(in, out, (long long)(bytes / 16), (hipStream_t)stream);
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return -1;
+  }
+  return 0;
+}
+
 // Events on the null stream, for callers timing their own regions.
 void* p3dfft_amd_event_create(void) {
   hipEvent_t e = nullptr;

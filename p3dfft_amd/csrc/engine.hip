@@ -395,6 +395,13 @@ __global__ void k_cheby(cpx<T>* o, long long ncol, int N, long long cs, long lon
   }
 }
 
+// plain streaming copy (16-byte vectors), used to measure achievable HBM
+// bandwidth and to calibrate traffic counters
+__global__ void k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
+}
+
 template <class T>
 __global__ void k_scale(cpx<T>* p, long long n, double s) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -681,7 +688,10 @@ void launch_col(const ColSpec& c, hipStream_t st) {
   }
   if (bytes > kLdsMax) throw Error{"column transform does not fit in LDS", -1};
   const int nt = pick_nt(elems, cpt);
-  if (c.fwd) dispatch_nt<GoCol<true>>(nt, a, (unsigned)nwg, bytes, st);
+  // DCT-I and DST-I (REDFT00 / RODFT00) are the same real transforms in
+  // ftran and btran: their extensions always go through the forward FFT
+  const bool fwd = c.fwd || c.kind == COL_DCT1 || c.kind == COL_DST1;
+  if (fwd) dispatch_nt<GoCol<true>>(nt, a, (unsigned)nwg, bytes, st);
   else dispatch_nt<GoCol<false>>(nt, a, (unsigned)nwg, bytes, st);
   check_hip(hipGetLastError(), "k_col launch");
 }
@@ -692,6 +702,13 @@ void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long 
   const int nt = 256;
   k_cheby<real_t><<<(unsigned)((ncol + nt - 1) / nt), nt, 0, st>>>(out, ncol, nzc, cs, ks, scale, L);
   check_hip(hipGetLastError(), "k_cheby launch");
+}
+
+void launch_copy16(const void* in, void* out, long long n16, hipStream_t st) {
+  if (n16 <= 0) return;
+  const long long blocks = std::min<long long>((n16 + 255) / 256, 256LL * 64);
+  k_copy16<<<(unsigned)blocks, 256, 0, st>>>((const uint4*)in, (uint4*)out, n16);
+  check_hip(hipGetLastError(), "k_copy16 launch");
 }
 
 void launch_scale(cplx_t* p, long long n, double s, hipStream_t st) {

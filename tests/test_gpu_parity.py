@@ -154,7 +154,7 @@ def test_device_api_overwrite_and_in_place(precision, ow):
     a = rng.standard_normal((nz, ny, nx))
     o = oplan(nx, ny, nz, ow=ow)
     ref_s = o.ftran_r2c(a)
-    ref_b, ref_in_after = o.btran_c2r(ref_s, "tff", keep_input=False)
+    ref_b, ref_in_after = o.btran_c2r(ref_s.copy(), "tff", keep_input=False)
     with lib_setup(precision, nx, ny, nz, ow=ow) as P:
         nb = max(P.real_bytes(), P.spec_bytes())
         buf = P.alloc(nb)
@@ -174,6 +174,13 @@ def test_device_api_overwrite_and_in_place(precision, ow):
 
 
 # ---------------------------------------------------------------- Chebyshev
+def cheby_tol(precision, nzc, Lz):
+    """The derivative recurrence d_k = sum_m (4/Lz) m a_m (m > k, m - k odd)
+    weighs each coefficient error by up to (4/Lz) nzc, so the FP32 bound is the
+    FFT bound times that factor; FP64 stays at 1e-12."""
+    return TOL[precision] if precision == 8 else TOL[4] * (4.0 / Lz) * nzc
+
+
 def cheby_points(nz, Lz):
     return np.cos(np.pi * np.arange(nz) / (nz - 1)) * 2.0 / Lz
 
@@ -188,7 +195,7 @@ def test_driver_cheby(precision):
     o = oplan(nx, ny, nz)
     with lib_setup(precision, nx, ny, nz) as P:
         c = P.cheby(a, Lz)
-        assert rel(c, o.cheby(a, Lz)) < TOL[precision]
+        assert rel(c, o.cheby(a, Lz)) < cheby_tol(precision, nz, Lz)
         c = c.astype(np.complex128)
         c[0] *= 2.0
         c[nz - 1] *= 2.0
@@ -214,7 +221,7 @@ def test_cheby_random_polynomial(precision, stride1):
     a = g[:, None, None] * xy[None]
     o = oplan(nx, ny, nz, stride1=stride1)
     with lib_setup(precision, nx, ny, nz, stride1=stride1) as P:
-        assert rel(P.cheby(a, Lz), o.cheby(a, Lz)) < TOL[precision]
+        assert rel(P.cheby(a, Lz), o.cheby(a, Lz)) < cheby_tol(precision, nz, Lz)
 
 
 # ---------------------------------------------------------------- errors, timers
