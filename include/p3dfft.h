@@ -100,6 +100,9 @@ void p3dfft_amd_clear_error(void);
 
 /* Stage timers on (default) or off. */
 void p3dfft_amd_set_timing(int on);
+/* Compile-time specialised kernels for the lengths that have them (default 1);
+ * 0 sends every length through the run-time-generic kernels. */
+void p3dfft_amd_set_specialised(int on);
 /* sizeof(p3dfft_real) of this build: 8 or 4. */
 int p3dfft_amd_precision(void);
 

@@ -69,6 +69,7 @@ def load(precision=8):
     L.p3dfft_amd_last_error.restype = ctypes.c_char_p
     L.p3dfft_amd_set_timing.argtypes = [ctypes.c_int]
     L.p3dfft_amd_precision.restype = ctypes.c_int
+    L.p3dfft_amd_set_specialised.argtypes = [ctypes.c_int]
     L.p3dfft_amd_malloc.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_malloc.restype = _vp
     L.p3dfft_amd_free.argtypes = [_vp]

@@ -519,6 +519,10 @@ void p3dfft_amd_set_timing(int on) {
 
 int p3dfft_amd_precision(void) { return (int)sizeof(real_t); }
 
+// Use the compile-time specialised kernels for the lengths that have them
+// (default 1); 0 runs every length through the run-time-generic kernels.
+void p3dfft_amd_set_specialised(int on) { set_specialised(on != 0); }
+
 // Device memory helpers for callers without their own HIP runtime bindings.
 void* p3dfft_amd_malloc(size_t bytes) {
   void* p = nullptr;

@@ -23,6 +23,7 @@
 #include <mutex>
 
 #include "engine.hpp"
+#include "engine_ct.hpp"
 
 namespace p3amd {
 
@@ -337,6 +338,169 @@ __global__ __launch_bounds__(NT) void k_col(ColArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// compile-time specialised kernels (same arguments and semantics as above)
+// ---------------------------------------------------------------------------
+template <class T, int B, int NT, class RL>
+__global__ __launch_bounds__(NT) void k_x_r2c_ct(XArgs a) {
+  typedef typename TwT<T>::type TwL;
+  constexpr int N = RL_N<RL>::value;
+  constexpr int BP = B + 1;
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const TwL* tw = (const TwL*)a.tw;
+  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
+  const T* in = (const T*)a.in;
+  {
+    // issue every load of the tile before the LDS writes
+    constexpr int LI = (B * N + NT - 1) / NT;
+    T xa[LI], xb[LI];
+#pragma unroll
+    for (int i = 0; i < LI; i++) {
+      const int q = threadIdx.x + i * NT;
+      const int b = q / N, j = q % N;
+      const long long ra = 2 * (pair0 + b), rb = ra + 1;
+      const bool ok = ((B * N) % NT == 0) || q < B * N;
+      xa[i] = (ok && ra < a.nrows) ? in[ra * a.si_row + j] : (T)0;
+      xb[i] = (ok && rb < a.nrows) ? in[rb * a.si_row + j] : (T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < LI; i++) {
+      const int q = threadIdx.x + i * NT;
+      if (((B * N) % NT == 0) || q < B * N) lds[(q % N) * BP + q / N] = mk<T>(xa[i], xb[i]);
+    }
+  }
+  __syncthreads();
+  run_fft_ct<T, true, B, BP, NT>(lds, tw, RL{});
+  cpx<T>* out = (cpx<T>*)a.out;
+  const T h = (T)0.5;
+  for (int q = threadIdx.x; q < B * a.nxo; q += NT) {
+    const int b = fdiv(q, a.dnxo), k = q - b * a.nxo;
+    const cpx<T> zk = lds[k * BP + b], zn = lds[(k == 0 ? 0 : N - k) * BP + b];
+    const long long ra = 2 * (pair0 + b), rb = ra + 1;
+    if (ra < a.nrows) out[ra * a.so_row + k] = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h);
+    if (rb < a.nrows) out[rb * a.so_row + k] = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
+  }
+}
+
+template <class T, int B, int NT, class RL>
+__global__ __launch_bounds__(NT) void k_x_c2r_ct(XArgs a) {
+  typedef typename TwT<T>::type TwL;
+  constexpr int N = RL_N<RL>::value;
+  constexpr int BP = B + 1;
+  constexpr int H = N / 2;
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const TwL* tw = (const TwL*)a.tw;
+  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
+  const cpx<T>* in = (const cpx<T>*)a.in;
+  {
+    constexpr int LI = (B * N + NT - 1) / NT;
+    cpx<T> va[LI], vb[LI];
+#pragma unroll
+    for (int i = 0; i < LI; i++) {
+      const int q = threadIdx.x + i * NT;
+      const int b = q / N, k = q % N;
+      const long long ra = 2 * (pair0 + b), rb = ra + 1;
+      const int kk = k <= H ? k : N - k;
+      const bool ok = (((B * N) % NT == 0) || q < B * N) && kk < a.nxo;
+      va[i] = (ok && ra < a.nrows) ? in[ra * a.si_row + kk] : mk<T>(0, 0);
+      vb[i] = (ok && rb < a.nrows) ? in[rb * a.si_row + kk] : mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < LI; i++) {
+      const int q = threadIdx.x + i * NT;
+      if (((B * N) % NT == 0) || q < B * N) {
+        const int b = q / N, k = q % N;
+        const int kk = k <= H ? k : N - k;
+        cpx<T> A = va[i], Bv = vb[i];
+        if (kk == 0 || 2 * kk == N) { A.y = 0; Bv.y = 0; }
+        lds[k * BP + b] = (k == kk) ? mk<T>(A.x - Bv.y, A.y + Bv.x) : mk<T>(A.x + Bv.y, Bv.x - A.y);
+      }
+    }
+  }
+  __syncthreads();
+  run_fft_ct<T, false, B, BP, NT>(lds, tw, RL{});
+  T* out = (T*)a.out;
+  for (int q = threadIdx.x; q < B * N; q += NT) {
+    const int b = q / N, j = q % N;
+    const long long ra = 2 * (pair0 + b), rb = ra + 1;
+    const cpx<T> z = lds[j * BP + b];
+    if (ra < a.nrows) out[ra * a.so_row + j] = z.x;
+    if (rb < a.nrows) out[rb * a.so_row + j] = z.y;
+  }
+}
+
+template <class T, bool FWD, int TX, int NT, class RL>
+__global__ __launch_bounds__(NT) void k_col_ct(ColArgs a) {
+  typedef typename TwT<T>::type TwL;
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int o = t / a.tiles;
+  const int ti = t - o * a.tiles;
+  const long long i0 = (long long)ti * TX;
+  const int nb = (int)((long long)a.n_inner - i0 < TX ? (long long)a.n_inner - i0 : TX);
+  const cpx<T>* in = (const cpx<T>*)a.in + i0 * a.si_in + (long long)o * a.so_in;
+  cpx<T>* out = (cpx<T>*)a.out + i0 * a.si_out + (long long)o * a.so_out;
+  const TwL* tw = (const TwL*)a.tw;
+  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  if (a.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast) {
+    // direct global <-> register first/last stages (the twiddle staging above
+    // is completed by the barrier at the end of the first stage)
+    ColIO<T> io;
+    io.in = in; io.out = out;
+    io.si_in = a.si_in; io.sl_in = a.sl_in; io.si_out = a.si_out; io.sl_out = a.sl_out;
+    io.nb = nb;
+    io.ld_lo = a.ld_lo; io.ld_hi = a.ld_hi; io.ld_shift = a.ld_shift;
+    io.nst = a.nst; io.st_lo = a.st_lo; io.st_shift = a.st_shift;
+    run_fft_io<T, FWD, TX, NT>(lds, tw, io, RL{});
+    return;
+  }
+  const int off = (a.kind == COL_DST1) ? 1 : 0;
+  for (int q = threadIdx.x; q < TX * a.nld; q += NT) {
+    int b, e;
+    if (a.ld_lenfast) { b = fdiv(q, a.dnld); e = q - b * a.nld; }
+    else { e = q / TX; b = q % TX; }
+    cpx<T> v = mk<T>(0, 0);
+    if (b < nb) {
+      const int src = e < a.ld_lo ? e : (e >= a.ld_hi ? e - a.ld_shift : -1);
+      if (src >= 0) v = in[(long long)b * a.si_in + (long long)src * a.sl_in];
+    }
+    lds[(e + off) * TX + b] = v;
+  }
+  __syncthreads();
+  if (a.kind == COL_DCT1) {
+    const int m = a.L.n;
+    for (int q = threadIdx.x; q < TX * (m - a.nld); q += NT) {
+      const int r = q / TX, b = q % TX, e = a.nld + r;
+      lds[e * TX + b] = lds[(m - e) * TX + b];
+    }
+    __syncthreads();
+  } else if (a.kind == COL_DST1) {
+    const int m = a.L.n;
+    for (int q = threadIdx.x; q < TX * (m - a.nld - 1); q += NT) {
+      const int r = q / TX, b = q % TX, e = a.nld + 1 + r;
+      if (e == a.nld + 1) lds[e * TX + b] = mk<T>(0, 0);
+      else { const cpx<T> u = lds[(m - e) * TX + b]; lds[e * TX + b] = mk<T>(-u.x, -u.y); }
+    }
+    for (int b = threadIdx.x; b < TX; b += NT) lds[b] = mk<T>(0, 0);
+    __syncthreads();
+  }
+  run_fft_ct<T, FWD, TX, TX, NT>(lds, tw, RL{});
+  for (int q = threadIdx.x; q < TX * a.nst; q += NT) {
+    int b, e;
+    if (a.st_lenfast) { b = fdiv(q, a.dnst); e = q - b * a.nst; }
+    else { e = q / TX; b = q % TX; }
+    if (b < nb) {
+      const int p = e < a.st_lo ? e : e + a.st_shift;
+      cpx<T> v;
+      if (a.kind == COL_DST1) { const cpx<T> u = lds[(p + 1) * TX + b]; v = mk<T>(-u.y, u.x); }
+      else v = lds[p * TX + b];
+      out[(long long)b * a.si_out + (long long)e * a.sl_out] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // small kernels
 // ---------------------------------------------------------------------------
 template <class TwL>
@@ -581,6 +745,95 @@ static void dispatch_nt(int nt, const A& a, unsigned g, size_t b, hipStream_t s)
   }
 }
 
+// ---- compile-time specialised lengths ----------------------------------
+constexpr int kEPTct = sizeof(real_t) == 4 ? 32 : 16;  // complex values per thread
+constexpr int pow2floor(int x) { int p = 1; while (p * 2 <= x) p *= 2; return p; }
+constexpr int ct_col_tx(int N) {
+  int tx = 128 / (int)sizeof(cplx_t);
+  int cap = pow2floor(256 * kEPTct / N);
+  if (cap < 1) cap = 1;
+  return tx < cap ? tx : cap;
+}
+constexpr int ct_nt(int elems) {
+  int nt = (elems / kEPTct + 63) / 64 * 64;
+  return nt < 64 ? 64 : (nt > 256 ? 256 : nt);
+}
+constexpr int ct_x_pairs(int N) {
+  int b = 16;
+  while (b > 1 && (b * N > 256 * kEPTct || (long long)(b + 1) * N * (long long)sizeof(cplx_t) > 72 * 1024)) b /= 2;
+  return b;
+}
+
+struct CtInfo {
+  int TX, NTC, BX, NTX, nr;
+  int radix[kMaxStages];
+  void (*col)(bool, const ColArgs&, unsigned, size_t, hipStream_t);
+  void (*xr2c)(const XArgs&, unsigned, size_t, hipStream_t);
+  void (*xc2r)(const XArgs&, unsigned, size_t, hipStream_t);
+};
+
+template <int... Rs>
+struct CtLen {
+  typedef RList<Rs...> RL;
+  static constexpr int N = RProd<Rs...>::value;
+  static constexpr int TX = ct_col_tx(N);
+  static constexpr int NTC = ct_nt(TX * N);
+  static constexpr int BX = ct_x_pairs(N);
+  static constexpr int NTX = ct_nt(BX * N);
+  static void col(bool fwd, const ColArgs& a, unsigned g, size_t bytes, hipStream_t s) {
+    if (fwd) {
+      static bool once = (allow_big_lds(k_col_ct<real_t, true, TX, NTC, RL>), true);
+      (void)once;
+      k_col_ct<real_t, true, TX, NTC, RL><<<g, NTC, bytes, s>>>(a);
+    } else {
+      static bool once = (allow_big_lds(k_col_ct<real_t, false, TX, NTC, RL>), true);
+      (void)once;
+      k_col_ct<real_t, false, TX, NTC, RL><<<g, NTC, bytes, s>>>(a);
+    }
+  }
+  static void xr2c(const XArgs& a, unsigned g, size_t bytes, hipStream_t s) {
+    static bool once = (allow_big_lds(k_x_r2c_ct<real_t, BX, NTX, RL>), true);
+    (void)once;
+    k_x_r2c_ct<real_t, BX, NTX, RL><<<g, NTX, bytes, s>>>(a);
+  }
+  static void xc2r(const XArgs& a, unsigned g, size_t bytes, hipStream_t s) {
+    static bool once = (allow_big_lds(k_x_c2r_ct<real_t, BX, NTX, RL>), true);
+    (void)once;
+    k_x_c2r_ct<real_t, BX, NTX, RL><<<g, NTX, bytes, s>>>(a);
+  }
+  static CtInfo info() {
+    CtInfo c;
+    std::memset(&c, 0, sizeof(c));
+    c.TX = TX; c.NTC = NTC; c.BX = BX; c.NTX = NTX;
+    const int rs[] = {Rs...};
+    c.nr = (int)sizeof...(Rs);
+    for (int i = 0; i < c.nr; i++) c.radix[i] = rs[i];
+    c.col = &col; c.xr2c = &xr2c; c.xc2r = &xc2r;
+    return c;
+  }
+};
+
+// Specialised kernel set for a length, if one exists and its radix schedule
+// matches the length plan (the twiddle tables are laid out by that schedule).
+static const CtInfo* find_ct(const Len1D& L) {
+  static const std::map<int, CtInfo> tab = {
+      {32, CtLen<8, 4>::info()},       {48, CtLen<4, 4, 3>::info()},    {64, CtLen<8, 8>::info()},
+      {96, CtLen<8, 4, 3>::info()},    {128, CtLen<8, 4, 4>::info()},   {160, CtLen<8, 4, 5>::info()},
+      {192, CtLen<8, 8, 3>::info()},   {256, CtLen<8, 8, 4>::info()},   {320, CtLen<8, 8, 5>::info()},
+      {384, CtLen<8, 4, 4, 3>::info()}, {512, CtLen<8, 8, 8>::info()},  {1024, CtLen<8, 8, 4, 4>::info()},
+      {2048, CtLen<8, 8, 8, 4>::info()}};
+  auto it = tab.find(L.d.n);
+  if (it == tab.end()) return nullptr;
+  const CtInfo& c = it->second;
+  if (c.nr != L.d.nst) return nullptr;
+  for (int i = 0; i < c.nr; i++)
+    if (c.radix[i] != L.d.radix[i]) return nullptr;
+  return &c;
+}
+
+static bool g_use_ct = true;
+void set_specialised(bool on) { g_use_ct = on; }
+
 // row-pair count per X tile
 static long long x_pairs_per_tile(int nx, long long pairs) {
   const long long tgt = (long long)(kTileBytes / sizeof(cplx_t));
@@ -607,6 +860,20 @@ static void x_setup(XArgs& a, const Len1D& L, long long nrows, int nx, size_t* b
   *nwg = (unsigned)g;
 }
 
+static void x_setup_ct(XArgs& a, const Len1D& L, const CtInfo& c, long long nrows, int nx, size_t* bytes,
+                       unsigned* nwg) {
+  const long long pairs = (nrows + 1) / 2;
+  a.B = c.BX;
+  a.L = L.d;
+  a.tw = L.tw;
+  const size_t data = (size_t)(c.BX + 1) * nx * sizeof(cplx_t);
+  *bytes = lds_layout(data, L, &a.tw_lds, &a.tw_off);
+  if (*bytes > 80 * 1024) { a.tw_lds = 0; a.tw_off = 0; *bytes = data; }
+  const long long g = (pairs + c.BX - 1) / c.BX;
+  if (g > 0x7fffffffLL) throw Error{"grid too large", -1};
+  *nwg = (unsigned)g;
+}
+
 void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_row, long long so_row, int nx,
                   int nxo, hipStream_t st) {
   if (nrows <= 0 || nxo <= 0) return;
@@ -615,6 +882,13 @@ void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_r
   std::memset(&a, 0, sizeof(a));
   a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;
   a.nxo = nxo; a.dnxo = make_fdiv(nxo);
+  if (const CtInfo* c = g_use_ct ? find_ct(L) : nullptr) {
+    size_t bytes; unsigned nwg;
+    x_setup_ct(a, L, *c, nrows, nx, &bytes, &nwg);
+    c->xr2c(a, nwg, bytes, st);
+    check_hip(hipGetLastError(), "k_x_r2c_ct launch");
+    return;
+  }
   size_t bytes; int nt; unsigned nwg;
   x_setup(a, L, nrows, nx, &bytes, &nt, &nwg);
   dispatch_nt<GoXr2c>(nt, a, nwg, bytes, st);
@@ -629,6 +903,13 @@ void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_r
   std::memset(&a, 0, sizeof(a));
   a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;
   a.nxo = nxi; a.dnxo = make_fdiv(nxi > 0 ? nxi : 1);
+  if (const CtInfo* c = g_use_ct ? find_ct(L) : nullptr) {
+    size_t bytes; unsigned nwg;
+    x_setup_ct(a, L, *c, nrows, nx, &bytes, &nwg);
+    c->xc2r(a, nwg, bytes, st);
+    check_hip(hipGetLastError(), "k_x_c2r_ct launch");
+    return;
+  }
   size_t bytes; int nt; unsigned nwg;
   x_setup(a, L, nrows, nx, &bytes, &nt, &nwg);
   dispatch_nt<GoXc2r>(nt, a, nwg, bytes, st);
@@ -660,6 +941,30 @@ void launch_col(const ColSpec& c, hipStream_t st) {
   }
   const int rows = std::max(nfft, c.nld + (c.kind == COL_DST1 ? 2 : 0));
   a.nrows_lds = rows;
+  a.ld_lenfast = (c.sl_in == 1 && c.si_in != 1) ? 1 : 0;
+  a.st_lenfast = (c.sl_out == 1 && c.si_out != 1) ? 1 : 0;
+  a.dnld = make_fdiv(c.nld);
+  a.dnst = make_fdiv(c.nst);
+  // DCT-I and DST-I (REDFT00 / RODFT00) are the same real transforms in
+  // ftran and btran: their extensions always go through the forward FFT
+  const bool fwd = c.fwd || c.kind == COL_DCT1 || c.kind == COL_DST1;
+  if (const CtInfo* ct = (L && g_use_ct) ? find_ct(*L) : nullptr) {
+    const int TX = ct->TX;
+    a.TX = TX;
+    a.dTX = make_fdiv(TX);
+    a.tiles = (c.n_inner + TX - 1) / TX;
+    const long long nwg = (long long)a.tiles * c.n_outer;
+    if (nwg > 0x7fffffffLL) throw Error{"grid too large", -1};
+    a.L = L->d;
+    a.tw = L->tw;
+    const size_t data = (size_t)TX * rows * sizeof(cplx_t);
+    size_t bytes = lds_layout(data, *L, &a.tw_lds, &a.tw_off);
+    if (bytes > 80 * 1024) { a.tw_lds = 0; a.tw_off = 0; bytes = data; }
+    if (bytes > kLdsMax) throw Error{"column transform does not fit in LDS", -1};
+    ct->col(fwd, a, (unsigned)nwg, bytes, st);
+    check_hip(hipGetLastError(), "k_col_ct launch");
+    return;
+  }
   const int cpt = L ? len_cap_per_thread(*L) : kEPT;
   // tile width: one 128-byte segment per row where capacity allows
   int TX = (int)(128 / sizeof(cplx_t));
@@ -669,10 +974,6 @@ void launch_col(const ColSpec& c, hipStream_t st) {
   while (TX > 1 && TX / 2 >= c.n_inner) TX /= 2;
   a.TX = TX;
   a.dTX = make_fdiv(TX);
-  a.ld_lenfast = (c.sl_in == 1 && c.si_in != 1) ? 1 : 0;
-  a.st_lenfast = (c.sl_out == 1 && c.si_out != 1) ? 1 : 0;
-  a.dnld = make_fdiv(c.nld);
-  a.dnst = make_fdiv(c.nst);
   a.tiles = (c.n_inner + TX - 1) / TX;
   const long long nwg = (long long)a.tiles * c.n_outer;
   if (nwg > 0x7fffffffLL) throw Error{"grid too large", -1};
@@ -688,9 +989,6 @@ void launch_col(const ColSpec& c, hipStream_t st) {
   }
   if (bytes > kLdsMax) throw Error{"column transform does not fit in LDS", -1};
   const int nt = pick_nt(elems, cpt);
-  // DCT-I and DST-I (REDFT00 / RODFT00) are the same real transforms in
-  // ftran and btran: their extensions always go through the forward FFT
-  const bool fwd = c.fwd || c.kind == COL_DCT1 || c.kind == COL_DST1;
   if (fwd) dispatch_nt<GoCol<true>>(nt, a, (unsigned)nwg, bytes, st);
   else dispatch_nt<GoCol<false>>(nt, a, (unsigned)nwg, bytes, st);
   check_hip(hipGetLastError(), "k_col launch");

@@ -1,0 +1,206 @@
+// engine_ct.hpp -- compile-time specialised Stockham engine.
+//
+// For the transform lengths the workloads use, the radix schedule, the tile
+// width and the thread count are template parameters, so every index
+// computation of a stage folds to shifts and constant multiplies and the
+// per-thread register arrays have fixed size.  The LDS tile holds B sequences
+// with element e of sequence b at lds[e*BP + b]; BP = B+1 pads each row so that
+// both the sequence-fastest accesses of the stages and the element-fastest
+// accesses of row loads avoid bank conflicts.  Stage twiddles use the same
+// table layout as the run-time engine (stage 0 has none; stage s holds
+// Ns*(R-1) entries W_{Ns R}^{k r}, k < Ns, r = 1..R-1).
+#pragma once
+#include "fft_device.hpp"
+
+namespace p3amd {
+
+template <int... Rs>
+struct RList {};
+
+template <int A, int... Rs>
+struct RProd {
+  static constexpr int value = A * RProd<Rs...>::value;
+};
+template <int A>
+struct RProd<A> {
+  static constexpr int value = A;
+};
+
+// One stage: radix R, Ns = product of the earlier radices, twiddles at Off.
+template <class T, int R, bool FWD, int N, int B, int BP, int NT, int Ns, int Off>
+__device__ __forceinline__ void stage_ct(cpx<T>* lds, const typename TwT<T>::type* tw) {
+  constexpr int nR = N / R;
+  constexpr int total = B * nR;
+  constexpr int IT = (total + NT - 1) / NT;
+  cpx<T> v[IT][R];
+#pragma unroll
+  for (int m = 0; m < IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if ((total % NT) == 0 || it < total) {
+      const int j = it / B, b = it % B;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[m][r] = lds[(j + r * nR) * BP + b];
+      if constexpr (Ns > 1) {
+        const int k = j % Ns;
+        const typename TwT<T>::type* w = tw + Off + k * (R - 1);
+#pragma unroll
+        for (int r = 1; r < R; r++) v[m][r] = twmul<FWD>(v[m][r], w[r - 1]);
+      }
+      dft<R, FWD>(v[m]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if ((total % NT) == 0 || it < total) {
+      const int j = it / B, b = it % B;
+      const int k = j % Ns;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) lds[(base + r * Ns) * BP + b] = v[m][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, bool FWD, int N, int B, int BP, int NT, int Ns, int Off, int R, int... Rest>
+__device__ __forceinline__ void stages_ct(cpx<T>* lds, const typename TwT<T>::type* tw) {
+  stage_ct<T, R, FWD, N, B, BP, NT, Ns, Off>(lds, tw);
+  if constexpr (sizeof...(Rest) > 0)
+    stages_ct<T, FWD, N, B, BP, NT, Ns * R, (Ns > 1 ? Off + Ns * (R - 1) : Off), Rest...>(lds, tw);
+}
+
+template <class T, bool FWD, int B, int BP, int NT, int... Rs>
+__device__ __forceinline__ void run_fft_ct(cpx<T>* lds, const typename TwT<T>::type* tw, RList<Rs...>) {
+  constexpr int N = RProd<Rs...>::value;
+  stages_ct<T, FWD, N, B, BP, NT, 1, 0, Rs...>(lds, tw);
+}
+
+// ---------------------------------------------------------------------------
+// Column transforms with the first stage fed straight from global memory and
+// the last stage stored straight to global memory (one LDS round trip fewer
+// at each end, and every thread keeps IT*R independent loads in flight).
+// The view is [len][inner] with the inner index contiguous; rows are remapped
+// on load (pruning zero-fill) and on store (pruning selection).
+// ---------------------------------------------------------------------------
+template <class T>
+struct ColIO {
+  const cpx<T>* in;
+  cpx<T>* out;
+  long long si_in, sl_in, si_out, sl_out;
+  int nb;                     // valid columns of this tile
+  int ld_lo, ld_hi, ld_shift; // LDS row e <- input row e (e<ld_lo), e-ld_shift (e>=ld_hi), else 0
+  int nst, st_lo, st_shift;   // output row e <- FFT row e (e<st_lo) or e+st_shift
+};
+
+template <class T, int R, bool FWD, int N, int B, int NT>
+__device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO<T>& io) {
+  constexpr int nR = N / R;
+  constexpr int total = B * nR;
+  constexpr int IT = (total + NT - 1) / NT;
+  cpx<T> v[IT][R];
+#pragma unroll
+  for (int m = 0; m < IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if ((total % NT) == 0 || it < total) {
+      const int j = it / B, b = it % B;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int e = j + r * nR;
+        const int src = e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1);
+        v[m][r] = (b < io.nb && src >= 0) ? io.in[(long long)b * io.si_in + (long long)src * io.sl_in]
+                                          : mk<T>(0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if ((total % NT) == 0 || it < total) {
+      const int j = it / B, b = it % B;
+      dft<R, FWD>(v[m]);
+#pragma unroll
+      for (int r = 0; r < R; r++) lds[(j * R + r) * B + b] = v[m][r];
+    }
+  }
+  __syncthreads();
+}
+
+// last stage: Ns * R == N, so every butterfly index j < Ns and k == j
+template <class T, int R, bool FWD, int N, int B, int NT, int Ns, int Off>
+__device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename TwT<T>::type* tw,
+                                              const ColIO<T>& io) {
+  static_assert(Ns * R == N, "last stage");
+  constexpr int total = B * Ns;
+  constexpr int IT = (total + NT - 1) / NT;
+#pragma unroll
+  for (int m = 0; m < IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if ((total % NT) == 0 || it < total) {
+      const int j = it / B, b = it % B;
+      cpx<T> v[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) v[r] = lds[(j + r * Ns) * B + b];
+      const typename TwT<T>::type* w = tw + Off + j * (R - 1);
+#pragma unroll
+      for (int r = 1; r < R; r++) v[r] = twmul<FWD>(v[r], w[r - 1]);
+      dft<R, FWD>(v);
+      if (b < io.nb) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int p = j + r * Ns;
+          const int e = p < io.st_lo ? p : p - io.st_shift;
+          if (p < io.st_lo || (p >= io.st_lo + io.st_shift && e < io.nst))
+            io.out[(long long)b * io.si_out + (long long)e * io.sl_out] = v[r];
+        }
+      }
+    }
+  }
+}
+
+template <class T, bool FWD, int N, int B, int NT, int Ns, int Off, int R, int... Rest>
+__device__ __forceinline__ void mid_and_last(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO<T>& io) {
+  if constexpr (sizeof...(Rest) == 0) {
+    stageL_global<T, R, FWD, N, B, NT, Ns, Off>(lds, tw, io);
+  } else {
+    stage_ct<T, R, FWD, N, B, B, NT, Ns, Off>(lds, tw);
+    mid_and_last<T, FWD, N, B, NT, Ns * R, (Ns > 1 ? Off + Ns * (R - 1) : Off), Rest...>(lds, tw, io);
+  }
+}
+
+template <class T, bool FWD, int B, int NT, int R0, int... Rs>
+__device__ __forceinline__ void run_fft_io(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO<T>& io,
+                                           RList<R0, Rs...>) {
+  static_assert(sizeof...(Rs) >= 1, "at least two stages");
+  constexpr int N = R0 * RProd<Rs...>::value;
+  stage0_global<T, R0, FWD, N, B, NT>(lds, io);
+  mid_and_last<T, FWD, N, B, NT, R0, 0, Rs...>(lds, tw, io);
+}
+
+// Twiddle table entries used by a radix list (all stages after the first).
+template <int... Rs>
+struct TwCount;
+template <int R0, int... Rs>
+struct TwCount<R0, Rs...> {
+  template <int Ns, int R, int... Rest>
+  static constexpr int tail() {
+    if constexpr (sizeof...(Rest) == 0) return Ns * (R - 1);
+    else return Ns * (R - 1) + tail<Ns * R, Rest...>();
+  }
+  static constexpr int value() {
+    if constexpr (sizeof...(Rs) == 0) return 0;
+    else return tail<R0, Rs...>();
+  }
+};
+
+}  // namespace p3amd
+
+namespace p3amd {
+template <class RL>
+struct RL_N;
+template <int... Rs>
+struct RL_N<RList<Rs...>> {
+  static constexpr int value = RProd<Rs...>::value;
+};
+}  // namespace p3amd

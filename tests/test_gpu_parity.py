@@ -110,6 +110,29 @@ def test_radix_coverage_round_trip(precision, shape):
     assert np.abs(back - a).max() < PREC[precision] * N * 0.25
 
 
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape", [(64, 48, 32), (96, 128, 160), (192, 40, 65), (384, 20, 33)])
+def test_generic_and_specialised_kernels_agree(precision, shape):
+    """Lengths with compile-time specialised kernels, run through both the
+    specialised and the run-time-generic engine, for every Z option."""
+    nx, ny, nz = shape
+    a = np.random.default_rng(7).standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz)
+    lib = p3dfft_amd.load(precision)
+    try:
+        for spec_on in (1, 0):
+            lib.p3dfft_amd_set_specialised(spec_on)
+            with lib_setup(precision, nx, ny, nz) as P:
+                for op in ("fft", "ffc", "ffs"):
+                    s = P.ftran_r2c(a, op)
+                    assert rel(s, o.ftran_r2c(a, op)) < TOL[precision], (op, spec_on)
+                    bop = op[2] + "ff"
+                    assert rel(P.btran_c2r(s, bop), o.btran_c2r(s.astype(np.complex128), bop)[0]) \
+                        < TOL[precision], (bop, spec_on)
+    finally:
+        lib.p3dfft_amd_set_specialised(1)
+
+
 # ---------------------------------------------------------------- pruning, layout, OW
 CUTS = [(12, 12, 10), (8, 12, 10), (6, 8, 6), (10, 9, 7), (12, 7, 8), (7, 12, 9)]
 
