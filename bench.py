@@ -59,8 +59,15 @@ def slot_bytes(slot, n, s):
     return 2 * nc * 2 * s
 
 
-def run_precision(precision, n, f, steps, warmup):
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def run_precision(precision, n, f, steps, warmup, chunk_bytes=None, timers=True):
     P = p3dfft_amd.P3DFFT(precision)
+    if chunk_bytes is not None:
+        P.lib.p3dfft_amd_set_chunk_bytes(chunk_bytes)
+    P.lib.p3dfft_amd_set_timing(1 if timers else 0)
     P.setup((1, 1), n, n, n)
     lib = P.lib
     try:
@@ -143,17 +150,31 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--chunk-mb", type=float, default=None,
+                    help="X/Y z-chunk size in MiB of intermediate (library default 0 = no chunking)")
+    ap.add_argument("--no-timers", action="store_true",
+                    help="record no per-stage HIP events (whole-step time only)")
+    ap.add_argument("--field", choices=["modes", "random"], default="modes",
+                    help="modes: the 8-mode + noise field; random: N(0,1) (fast, for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the multi-GPU exchange is a design, "
               "not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     n = args.n
-    f = modes_field(n, 2026)
-    r32 = run_precision(4, n, f, args.steps, args.warmup)
-    r64 = None if args.no_fp64 else run_precision(8, n, f, args.steps, args.warmup)
+    log("field %s %d^3" % (args.field, n))
+    f = modes_field(n, 2026) if args.field == "modes" else np.random.default_rng(1).standard_normal((n, n, n))
+    cb = None if args.chunk_mb is None else int(args.chunk_mb * (1 << 20))
+    log("fp32 leg")
+    r32 = run_precision(4, n, f, args.steps, args.warmup, cb, not args.no_timers)
+    log("fp32 %.4f ms/step" % r32["ms_per_step"])
+    r64 = None
+    if not args.no_fp64:
+        r64 = run_precision(8, n, f, args.steps, args.warmup, cb, not args.no_timers)
+        log("fp64 %.4f ms/step" % r64["ms_per_step"])
     lib = p3dfft_amd.load(4)
     cpk = copy_peak(lib)
+    log("copy peak %.1f GB/s" % (cpk / 1e9))
 
     s = 4
     nr, nc = n ** 3, (n // 2 + 1) * n * n
@@ -180,7 +201,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic: 8 random-phase Fourier modes + N(0,1e-3) noise, seed 2026",
         "config": {"workload": "%d^3 r2c 'fft' + c2r 'tff', FP32 bulk, compensated twiddles, 1x1 grid, "
-                               "fields resident in HBM" % n, "grid": [n, n, n], "op": ["fft", "tff"]},
+                               "fields resident in HBM" % n, "grid": [n, n, n], "op": ["fft", "tff"],
+                   "xy_chunk_mib": 0 if args.chunk_mb is None else args.chunk_mb},
         "roofline": {"bound": "hbm", "kernel": SLOT_NAMES[dom_slot], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "bytes_per_launch": slot_bytes(dom_slot, n, s), "avg_launch_ms": dom_avg * 1e3},
@@ -197,6 +219,7 @@ def main():
         b64 = r64["back"].astype(np.float64)
         line["rel_l2_roundtrip_fp32_vs_fp64"] = float(np.linalg.norm(r32["back"] - b64) / np.linalg.norm(b64))
     if not args.no_cpu:
+        log("cpu baseline (oracle, %d threads)" % args.cpu_threads)
         line["cpu_baseline"] = cpu_baseline(n, f, args.cpu_threads)
     print(json.dumps(line))
 

@@ -87,6 +87,27 @@ int p3dfft_ftran_r2c_dev(const p3dfft_real *in, p3dfft_real *out, const unsigned
 int p3dfft_btran_c2r_dev(p3dfft_real *in, p3dfft_real *out, const unsigned char *op, void *stream);
 int p3dfft_cheby_dev(const p3dfft_real *in, p3dfft_real *out, p3dfft_real Lz, void *stream);
 
+/* ---- share mode: one rank of a P1 x P2 grid (no counterpart entry points in
+ * the reference; they split build/ftran.F90:489-780 and build/btran.F90:396-679
+ * at the MPI_Alltoallv calls of fcomm1/fcomm2/bcomm1/bcomm2).  Buffers are
+ * device pointers in complex elements laid out as the reference's send and
+ * receive buffers; the caller moves them between ranks. ----------------- */
+int p3dfft_setup_share(int *dims, int *ipid, int *jpid, int *nx, int *ny, int *nz, int *nxc, int *nyc, int *nzc,
+                       int *ow, int *memsize);
+/* per-peer offsets/counts (complex elements): forward row exchange (p1 peers),
+ * forward column exchange (p2 peers); backward exchanges swap send and recv */
+int p3dfft_share_counts(long long *fsnd_st, long long *fsnd_ct, long long *frcv_st, long long *frcv_ct,
+                        long long *ksnd_st, long long *ksnd_ct, long long *krcv_st, long long *krcv_ct);
+/* which: 0 fcomm1 pack, 1 fcomm1 unpack, 2 fcomm2 pack, 3 bcomm1 unpack,
+ * 4 bcomm2 pack, 5 bcomm2 unpack */
+int p3dfft_share_pack(int which, const p3dfft_real *src, p3dfft_real *dst, void *stream);
+int p3dfft_share_ftran_x(const p3dfft_real *in, p3dfft_real *send1, void *stream);
+int p3dfft_share_ftran_y(const p3dfft_real *recv1, p3dfft_real *send2, void *stream);
+int p3dfft_share_ftran_z(p3dfft_real *recv2, p3dfft_real *out, const unsigned char *op, void *stream);
+int p3dfft_share_btran_z(p3dfft_real *in, p3dfft_real *send, const unsigned char *op, void *stream);
+int p3dfft_share_btran_y(const p3dfft_real *recv, p3dfft_real *send, void *stream);
+int p3dfft_share_btran_x(const p3dfft_real *recv, p3dfft_real *out, void *stream);
+
 /* Select the (Z,Y,X) complex layout for the next p3dfft_setup: the
  * reference's compile-time STRIDE1 option as a run-time switch. */
 void p3dfft_amd_set_stride1(int on);
@@ -98,6 +119,12 @@ void p3dfft_amd_set_error_mode(int mode);
 const char *p3dfft_amd_last_error(void);
 void p3dfft_amd_clear_error(void);
 
+/* Bytes of X/Y intermediate per z-plane chunk (default 0 = one chunk):
+ * each chunk's intermediate is re-read while it is still in the last-level cache. */
+void p3dfft_amd_set_chunk_bytes(size_t bytes);
+/* Bytes of X/Y intermediate per z-plane chunk (default 0 = one chunk):
+ * each chunk's intermediate is re-read while it is still in the last-level cache. */
+void p3dfft_amd_set_chunk_bytes(size_t bytes);
 /* Stage timers on (default) or off. */
 void p3dfft_amd_set_timing(int on);
 /* Compile-time specialised kernels for the lengths that have them (default 1);

@@ -70,6 +70,8 @@ def load(precision=8):
     L.p3dfft_amd_set_timing.argtypes = [ctypes.c_int]
     L.p3dfft_amd_precision.restype = ctypes.c_int
     L.p3dfft_amd_set_specialised.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_set_chunk_bytes.argtypes = [ctypes.c_size_t]
+    L.p3dfft_amd_set_chunk_bytes.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_malloc.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_malloc.restype = _vp
     L.p3dfft_amd_free.argtypes = [_vp]
@@ -86,6 +88,19 @@ def load(precision=8):
     L.p3dfft_amd_event_elapsed_ms.argtypes = [_vp, _vp]
     L.p3dfft_amd_event_elapsed_ms.restype = ctypes.c_float
     L.p3dfft_amd_event_destroy.argtypes = [_vp]
+    _ip8 = ctypes.POINTER(ctypes.c_longlong)
+    L.p3dfft_setup_share.argtypes = [_ip] * 11
+    L.p3dfft_setup_share.restype = ctypes.c_int
+    L.p3dfft_share_counts.argtypes = [_ip8] * 8
+    L.p3dfft_share_counts.restype = ctypes.c_int
+    L.p3dfft_share_pack.argtypes = [ctypes.c_int, _vp, _vp, _vp]
+    L.p3dfft_share_pack.restype = ctypes.c_int
+    for n in ("p3dfft_share_ftran_x", "p3dfft_share_ftran_y", "p3dfft_share_btran_y", "p3dfft_share_btran_x"):
+        getattr(L, n).argtypes = [_vp, _vp, _vp]
+        getattr(L, n).restype = ctypes.c_int
+    for n in ("p3dfft_share_ftran_z", "p3dfft_share_btran_z"):
+        getattr(L, n).argtypes = [_vp, _vp, ctypes.c_char_p, _vp]
+        getattr(L, n).restype = ctypes.c_int
     if L.p3dfft_amd_precision() != precision:
         raise P3DFFTError("library %s reports precision %d" % (path, L.p3dfft_amd_precision()))
     _libs[precision] = L
@@ -258,3 +273,93 @@ def _op(op):
 
 def device_count():
     return load(8).p3dfft_amd_device_count()
+
+
+PACK_FCOMM1, UNPACK_FCOMM1, PACK_FCOMM2, UNPACK_BCOMM1, PACK_BCOMM2, UNPACK_BCOMM2 = range(6)
+
+
+class Share(P3DFFT):
+    """One rank (ipid, jpid) of a p1 x p2 processor grid ("share mode"): the
+    reference's per-rank stages with the exchange buffers exposed.  Buffers
+    are DeviceBuffers holding complex elements in the reference's layouts."""
+
+    def setup_share(self, dims, ipid, jpid, nx, ny, nz, nxc=None, nyc=None, nzc=None, overwrite=True):
+        L = self.lib
+        L.p3dfft_amd_set_error_mode(1)
+        L.p3dfft_amd_clear_error()
+        L.p3dfft_amd_set_stride1(0)
+        d = (ctypes.c_int * 2)(*dims)
+        ms = (ctypes.c_int * 3)()
+        vals = [ipid, jpid, nx, ny, nz, nxc or nx, nyc or ny, nzc or nz, 1 if overwrite else 0]
+        cv = [ctypes.c_int(v) for v in vals]
+        rc = L.p3dfft_setup_share(d, *[ctypes.byref(c) for c in cv], ms)
+        self._raise_if_error(force=rc != 0)
+        self.active = True
+        self.dims, self.ipid, self.jpid = tuple(dims), ipid, jpid
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nxc, self.nyc, self.nzc = nxc or nx, nyc or ny, nzc or nz
+        self.stride1 = False
+        self.nxhpc = self.nxc // 2 + 1
+        self.nxhp = nx // 2 + 1
+        self.memsize = tuple(ms)
+        s1, _, z1 = self.get_dims(1)
+        s2, _, z2 = self.get_dims(2)
+        self.jisize, self.kjsize = z1[1], z1[2]
+        self.iisize, self.jjsize = z2[0], z2[1]
+        return self.memsize
+
+    def counts(self):
+        p1, p2 = self.dims
+        arrs = [(ctypes.c_longlong * n)() for n in (p1, p1, p1, p1, p2, p2, p2, p2)]
+        if self.lib.p3dfft_share_counts(*arrs):
+            self._raise_if_error(force=True)
+        keys = ("fsnd_st", "fsnd_ct", "frcv_st", "frcv_ct", "ksnd_st", "ksnd_ct", "krcv_st", "krcv_ct")
+        return {k: list(a) for k, a in zip(keys, arrs)}
+
+    # element counts of the stage arrays
+    def n_real(self):
+        return self.nx * self.jisize * self.kjsize
+
+    def n_buf2(self):
+        return self.nxhp * self.jisize * self.kjsize
+
+    def n_ypen(self):
+        return self.iisize * self.ny * self.kjsize
+
+    def n_send1(self):
+        return self.nxhpc * self.jisize * self.kjsize
+
+    def n_send2(self):
+        return self.iisize * self.nyc * self.kjsize
+
+    def n_zpen(self):
+        return self.iisize * self.jjsize * self.nz
+
+    def n_spec(self):
+        return self.iisize * self.jjsize * self.nzc
+
+    def pack(self, which, src, dst, stream=None):
+        if self.lib.p3dfft_share_pack(which, src.ptr, dst.ptr, stream):
+            self._raise_if_error(force=True)
+
+    def _call(self, name, *args):
+        if getattr(self.lib, name)(*args):
+            self._raise_if_error(force=True)
+
+    def ftran_x(self, inp, send1, stream=None):
+        self._call("p3dfft_share_ftran_x", inp.ptr, send1.ptr, stream)
+
+    def ftran_y(self, recv1, send2, stream=None):
+        self._call("p3dfft_share_ftran_y", recv1.ptr, send2.ptr, stream)
+
+    def ftran_z(self, recv2, out, op="fft", stream=None):
+        self._call("p3dfft_share_ftran_z", recv2.ptr, out.ptr, _op(op), stream)
+
+    def btran_z(self, inp, send, op="tff", stream=None):
+        self._call("p3dfft_share_btran_z", inp.ptr, send.ptr, _op(op), stream)
+
+    def btran_y(self, recv, send, stream=None):
+        self._call("p3dfft_share_btran_y", recv.ptr, send.ptr, stream)
+
+    def btran_x(self, recv, out, stream=None):
+        self._call("p3dfft_share_btran_x", recv.ptr, out.ptr, stream)

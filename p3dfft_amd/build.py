@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "bin", "amdflang")
 
-SOURCES = ["engine.hip", "api.cpp"]
+SOURCES = ["engine.hip", "share.hip", "api.cpp", "api_share.cpp"]
 # packed-FP32 instructions are disabled for all device code
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -41,7 +41,7 @@ def _run(cmd):
 def _compile_obj(src, defs, tag):
     obj = os.path.join(LIB, "obj", tag, os.path.splitext(src)[0] + ".o")
     os.makedirs(os.path.dirname(obj), exist_ok=True)
-    lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
+    lang = ["-x", "hip"]
     _run([HIPCC] + lang + HIP_FLAGS + defs + ["-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj])
     return obj
 

@@ -7,6 +7,7 @@
 // local (build/ftran.F90:539,595 and build/btran.F90:430,646 take the iproc=1 /
 // jproc=1 branches), so they are folded into the load/store indexing of the
 // neighbouring FFT kernels.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -180,6 +181,16 @@ static int zkind(char c, bool* none) {
   return -1;
 }
 
+// z-planes per X/Y chunk: the chunk intermediate is sized to stay resident in
+// the last-level cache; small problems run as one chunk
+int chunk_planes() {
+  const size_t plane = (size_t)g.nxhpc * g.ny * sizeof(cplx_t);
+  if (g.chunk_bytes == 0 || plane * (size_t)g.nz <= 2 * g.chunk_bytes) return g.nz;
+  size_t zc = g.chunk_bytes / plane;
+  if (zc < 1) zc = 1;
+  return (int)std::min<size_t>(zc, (size_t)g.nz);
+}
+
 // ---------------------------------------------------------------------------
 // forward: X r2c -> Y c2c (per z-plane) -> Z op, build/ftran.F90:489-780
 // ---------------------------------------------------------------------------
@@ -190,23 +201,37 @@ bool forward_dev(const real_t* in, cplx_t* out, const unsigned char* op, hipStre
   const int nx = g.nx, ny = g.ny, nz = g.nz, nxhpc = g.nxhpc, nyc = g.nyc, nzc = g.nzc;
   const int dny = ny - nyc, dnz = nz - nzc;
   const long long nxy = (long long)nxhpc * nyc;
-  // X r2c into w1 (nxhpc, ny, nz); seg_copy_x folded into the store
-  tstart(5, s);
-  launch_x_r2c(in, g.w1, (long long)ny * nz, nx, nxhpc, nx, nxhpc, s);
-  tstop(5, s);
-  // Y c2c; seg_copy_y (keep y < nycph, then y + dny) folded into the store
+  // X r2c then Y c2c, z-plane chunk by chunk: the (nxhpc, ny, zc) intermediate
+  // of a chunk is re-read by the Y kernel right after the X kernel wrote it,
+  // so it is served from the last-level cache instead of HBM.
+  // seg_copy_x and seg_copy_y (keep y < nycph, then y + dny) are folded into
+  // the stores.
   const bool direct = (dnz == 0 && !g.stride1);
   cplx_t* ydst = direct ? out : g.w2;
-  ColSpec y{};
-  y.in = g.w1; y.out = ydst;
-  y.si_in = 1; y.sl_in = nxhpc; y.so_in = (long long)nxhpc * ny;
-  y.si_out = 1; y.sl_out = nxhpc; y.so_out = nxy;
-  y.n_inner = nxhpc; y.n_outer = nz; y.n = ny; y.kind = COL_C2C; y.fwd = true;
-  y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
-  y.nst = nyc; y.st_lo = g.nycph; y.st_shift = dny;
-  tstart(7, s);
-  launch_col(y, s);
-  tstop(7, s);
+  // a complex output plane is larger than a real input plane: when the output
+  // overlaps the input (in-place use) a chunk would overwrite input planes of
+  // later chunks, so such calls run as one chunk
+  const char* ib = (const char*)in;
+  const char* ob = (const char*)ydst;
+  const bool alias = ob < ib + (size_t)nx * ny * nz * sizeof(real_t) &&
+                     ib < ob + (size_t)nxhpc * nyc * nz * sizeof(cplx_t);
+  const int zc = alias ? nz : chunk_planes();
+  for (int z0 = 0; z0 < nz; z0 += zc) {
+    const int nzk = std::min(zc, nz - z0);
+    tstart(5, s);
+    launch_x_r2c(in + (size_t)z0 * ny * nx, g.w1, (long long)ny * nzk, nx, nxhpc, nx, nxhpc, s);
+    tstop(5, s);
+    ColSpec y{};
+    y.in = g.w1; y.out = ydst + (size_t)z0 * nxy;
+    y.si_in = 1; y.sl_in = nxhpc; y.so_in = (long long)nxhpc * ny;
+    y.si_out = 1; y.sl_out = nxhpc; y.so_out = nxy;
+    y.n_inner = nxhpc; y.n_outer = nzk; y.n = ny; y.kind = COL_C2C; y.fwd = true;
+    y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
+    y.nst = nyc; y.st_lo = g.nycph; y.st_shift = dny;
+    tstart(7, s);
+    launch_col(y, s);
+    tstop(7, s);
+  }
   // Z op; seg_copy_z (keep z < nzcph, then z + dnz) and the STRIDE1 layout
   // folded into the store
   tstart(8, s);
@@ -278,21 +303,26 @@ bool backward_dev(cplx_t* in, real_t* out, const unsigned char* op, hipStream_t 
     zdst = g.w2;
   }
   tstop(9, s);
-  // Y c2c backward; seg_copy_y + seg_zero_y folded into the load
-  ColSpec y{};
-  y.in = zdst; y.out = g.w1;
-  y.si_in = 1; y.sl_in = nxhpc; y.so_in = nxy;
-  y.si_out = 1; y.sl_out = nxhpc; y.so_out = (long long)nxhpc * ny;
-  y.n_inner = nxhpc; y.n_outer = nz; y.n = ny; y.kind = COL_C2C; y.fwd = false;
-  y.nld = ny; y.ld_lo = g.nycph; y.ld_hi = g.stride1 ? g.nycph + dny : ny - g.nycph; y.ld_shift = dny;
-  y.nst = ny; y.st_lo = ny; y.st_shift = 0;
-  tstart(10, s);
-  launch_col(y, s);
-  tstop(10, s);
-  // X c2r; seg_copy_x + seg_zero_x (x > nxhpc) folded into the load
-  tstart(12, s);
-  launch_x_c2r(g.w1, out, (long long)ny * nz, nxhpc, nx, nx, nxhpc, s);
-  tstop(12, s);
+  // Y c2c backward then X c2r, chunk by chunk over z-planes (the chunk's
+  // intermediate stays in the last-level cache); seg_copy_y + seg_zero_y and
+  // seg_copy_x + seg_zero_x (x > nxhpc) are folded into the loads
+  const int zc = chunk_planes();
+  for (int z0 = 0; z0 < nz; z0 += zc) {
+    const int nzk = std::min(zc, nz - z0);
+    ColSpec y{};
+    y.in = zdst + (size_t)z0 * nxy; y.out = g.w1;
+    y.si_in = 1; y.sl_in = nxhpc; y.so_in = nxy;
+    y.si_out = 1; y.sl_out = nxhpc; y.so_out = (long long)nxhpc * ny;
+    y.n_inner = nxhpc; y.n_outer = nzk; y.n = ny; y.kind = COL_C2C; y.fwd = false;
+    y.nld = ny; y.ld_lo = g.nycph; y.ld_hi = g.stride1 ? g.nycph + dny : ny - g.nycph; y.ld_shift = dny;
+    y.nst = ny; y.st_lo = ny; y.st_shift = 0;
+    tstart(10, s);
+    launch_col(y, s);
+    tstop(10, s);
+    tstart(12, s);
+    launch_x_c2r(g.w1, out + (size_t)z0 * ny * nx, (long long)ny * nzk, nxhpc, nx, nx, nxhpc, s);
+    tstop(12, s);
+  }
   return true;
 }
 
@@ -315,8 +345,17 @@ bool require_setup() {
   return true;
 }
 
+// the whole-transform entry points run the 1x1 grid only
+static bool require_full() {
+  if (!require_setup()) return false;
+  if (g.share) return fatal(" p3dfft_amd: a share-mode setup runs the p3dfft_share_* stages only");
+  return true;
+}
+
 void release_state() {
   timers_resolve();
+  share_release();
+  g.share = false;
   for (auto e : g.evpool) (void)hipEventDestroy(e);
   g.evpool.clear();
   if (g.w1) (void)hipFree(g.w1);
@@ -417,7 +456,7 @@ void p3dfft_get_dims(int* start, int* end, int* size, int* conf) {
 
 // build/ftran.F90:469-485 (host arrays; copied to and from HBM inside the call)
 void p3dfft_ftran_r2c(real_t* A, real_t* B, unsigned char* op) {
-  if (!require_setup()) return;
+  if (!require_full()) return;
   GUARD_BEGIN
   ensure_staging();
   check_hip(hipMemcpy(g.stage_a, A, real_elems() * sizeof(real_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
@@ -428,7 +467,7 @@ void p3dfft_ftran_r2c(real_t* A, real_t* B, unsigned char* op) {
 
 // build/btran.F90:377-392
 void p3dfft_btran_c2r(real_t* A, real_t* B, unsigned char* op) {
-  if (!require_setup()) return;
+  if (!require_full()) return;
   GUARD_BEGIN
   ensure_staging();
   check_hip(hipMemcpy(g.stage_a, A, spec_elems() * sizeof(cplx_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
@@ -439,7 +478,7 @@ void p3dfft_btran_c2r(real_t* A, real_t* B, unsigned char* op) {
 
 // build/ftran.F90:359-379
 void p3dfft_cheby(real_t* A, real_t* B, real_t* Lz) {
-  if (!require_setup()) return;
+  if (!require_full()) return;
   GUARD_BEGIN
   ensure_staging();
   check_hip(hipMemcpy(g.stage_a, A, real_elems() * sizeof(real_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
@@ -478,7 +517,7 @@ void set_timers(void) {
 // `stream` (a hipStream_t, or null for the default stream) and the call
 // returns without synchronising.
 int p3dfft_ftran_r2c_dev(const real_t* in, real_t* out, const unsigned char* op, void* stream) {
-  if (!require_setup()) return -1;
+  if (!require_full()) return -1;
   GUARD_BEGIN
   return forward_dev(in, (cplx_t*)out, op, (hipStream_t)stream) ? 0 : -1;
   GUARD_END
@@ -486,7 +525,7 @@ int p3dfft_ftran_r2c_dev(const real_t* in, real_t* out, const unsigned char* op,
 }
 
 int p3dfft_btran_c2r_dev(real_t* in, real_t* out, const unsigned char* op, void* stream) {
-  if (!require_setup()) return -1;
+  if (!require_full()) return -1;
   GUARD_BEGIN
   return backward_dev((cplx_t*)in, out, op, (hipStream_t)stream) ? 0 : -1;
   GUARD_END
@@ -494,7 +533,7 @@ int p3dfft_btran_c2r_dev(real_t* in, real_t* out, const unsigned char* op, void*
 }
 
 int p3dfft_cheby_dev(const real_t* in, real_t* out, real_t Lz, void* stream) {
-  if (!require_setup()) return -1;
+  if (!require_full()) return -1;
   GUARD_BEGIN
   return cheby_dev(in, (cplx_t*)out, (double)Lz, (hipStream_t)stream) ? 0 : -1;
   GUARD_END
@@ -510,6 +549,9 @@ void p3dfft_amd_set_stride1(int on) { g.stride1_next = (on != 0); }
 void p3dfft_amd_set_error_mode(int mode) { g_err_mode = mode; }
 const char* p3dfft_amd_last_error(void) { return g_last_error.c_str(); }
 void p3dfft_amd_clear_error(void) { g_last_error.clear(); }
+
+// Bytes of the X/Y intermediate processed per chunk (0 = no chunking).
+void p3dfft_amd_set_chunk_bytes(size_t bytes) { g.chunk_bytes = bytes; }
 
 // Stage timers on/off (default on).
 void p3dfft_amd_set_timing(int on) {

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "share.hpp"
 
 namespace p3amd {
 
@@ -29,10 +30,17 @@ struct State {
   // HBM work buffers: w1 (nxhpc, ny, nz), w2 (nxhpc, nyc, nz)
   cplx_t* w1 = nullptr;
   cplx_t* w2 = nullptr;
+  // share mode (one rank of a P1 x P2 grid): Z pencil work buffer and splits
+  bool share = false;
+  cplx_t* w3 = nullptr;
+  ShareGeom sg;
   // staging of host arrays for the host-pointer entry points
   void* stage_a = nullptr;
   void* stage_b = nullptr;
   size_t stage_bytes = 0;
+  // X/Y chunk size in bytes of intermediate, 0 = whole grid in one pass
+  // (measured on MI355X at 512^3: chunks of 16-128 MiB were 10-110% slower)
+  size_t chunk_bytes = 0;
   hipStream_t stream = nullptr;
   // 12 timer slots (seconds)
   double timers[12] = {0};
@@ -60,5 +68,7 @@ bool forward_dev(const real_t* in, cplx_t* out, const unsigned char* op, hipStre
 bool backward_dev(cplx_t* in, real_t* out, const unsigned char* op, hipStream_t s);
 bool cheby_dev(const real_t* in, cplx_t* out, double Lz, hipStream_t s);
 void release_state();
+int chunk_planes();
+void share_release();
 
 }  // namespace p3amd
