@@ -141,6 +141,62 @@ def cpu_baseline(n, f, threads):
                       % (n, dt)}
 
 
+def fp64_poisson(f):
+    """FP64 numpy solve of -f_hat/|k|^2 (k=0 -> 0): the check of served results."""
+    nz, ny, nx = f.shape
+    F = np.fft.rfftn(f.astype(np.float64))
+    kx = np.arange(nx // 2 + 1)
+    ky = np.fft.fftfreq(ny, 1.0 / ny)
+    kz = np.fft.fftfreq(nz, 1.0 / nz)
+    k2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    mult = np.where(k2 > 0, -1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
+    return np.fft.irfftn(F * mult, s=(nz, ny, nx), axes=(0, 1, 2))
+
+
+def serve_leg(n_req, slots, load_frac=0.7):
+    """Mixed-size spectral Poisson solves (grid sizes drawn per axis from
+    {32,...,384}), FP32 with compensated twiddles, fields resident in HBM.
+    Throughput at saturation (all requests queued at t=0) for the served loop
+    and for one-at-a-time solves; p50/p99 latency of the served loop with
+    exponential inter-arrival times at load_frac of its capacity."""
+    from p3dfft_amd import serve
+    shapes = serve.make_requests(n_req)
+    S = serve.PoissonServer(shapes, precision=4, slots=slots)
+    try:
+        order = list(range(n_req))
+        S.fill()
+        # accuracy on a sample, against an FP64 solve of the same right-hand side
+        sample = [i for i in order if np.prod(shapes[i]) <= 128 ** 3][:8]
+        rhs = {i: S.download(i) for i in sample}
+        for i in sample:
+            S.solve_alone(i)
+        err = max(float(np.linalg.norm(S.download(i) - fp64_poisson(rhs[i])) / np.linalg.norm(fp64_poisson(rhs[i])))
+                  for i in sample)
+        S.fill()
+        S.run(order[:min(50, n_req)], np.zeros(min(50, n_req)))  # warm
+        S.fill()
+        _, mk_served, peak = S.run(order, np.zeros(n_req))
+        S.fill()
+        _, mk_seq, _ = S.run(order, np.zeros(n_req), sequential=True)
+        thr_served, thr_seq = n_req / mk_served, n_req / mk_seq
+        S.fill()
+        arr = serve.exp_arrivals(n_req, 1.0 / (load_frac * thr_served))
+        lat, _, _ = S.run(order, arr)
+        S.fill()
+        lat_q, _, _ = S.run(order, arr, sequential=True)
+    finally:
+        S.close()
+    return {"requests": n_req, "slots": slots, "solves_per_s": thr_served, "one_at_a_time_solves_per_s": thr_seq,
+            "speedup_vs_one_at_a_time": thr_served / thr_seq,
+            "latency_load": "exponential inter-arrival at %.0f%% of served capacity (%.0f req/s)"
+                            % (100 * load_frac, load_frac * thr_served),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "one_at_a_time_p50_ms": float(np.percentile(lat_q, 50) * 1e3),
+            "one_at_a_time_p99_ms": float(np.percentile(lat_q, 99) * 1e3),
+            "peak_work_bytes": int(peak), "rel_l2_vs_fp64_max": err,
+            "mean_points_per_request": float(np.mean([np.prod(s) for s in shapes]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +210,9 @@ def main():
                     help="X/Y z-chunk size in MiB of intermediate (library default 0 = no chunking)")
     ap.add_argument("--no-timers", action="store_true",
                     help="record no per-stage HIP events (whole-step time only)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the served Poisson leg (0 = skip)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--field", choices=["modes", "random"], default="modes",
                     help="modes: the 8-mode + noise field; random: N(0,1) (fast, for profiler runs)")
     args = ap.parse_args()
@@ -218,6 +277,11 @@ def main():
         line["rel_l2_spectrum_fp32_vs_fp64"] = float(np.linalg.norm(r32["spec"] - ref) / np.linalg.norm(ref))
         b64 = r64["back"].astype(np.float64)
         line["rel_l2_roundtrip_fp32_vs_fp64"] = float(np.linalg.norm(r32["back"] - b64) / np.linalg.norm(b64))
+    if args.serve_requests > 0:
+        log("serve leg (%d requests)" % args.serve_requests)
+        line["serve"] = serve_leg(args.serve_requests, args.serve_slots)
+        log("serve %.1f solves/s (one-at-a-time %.1f)" % (line["serve"]["solves_per_s"],
+                                                          line["serve"]["one_at_a_time_solves_per_s"]))
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % args.cpu_threads)
         line["cpu_baseline"] = cpu_baseline(n, f, args.cpu_threads)

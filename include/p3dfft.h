@@ -50,6 +50,13 @@ void p3dfft_btran_c2r(p3dfft_real *A, p3dfft_real *B, unsigned char *op);
  * Chebyshev derivative recurrence along Z. */
 void p3dfft_cheby(p3dfft_real *A, p3dfft_real *B, p3dfft_real *Lz);
 
+/* build/ftran.F90:84 (p3dfft_ftran_r2c_many_w), build/btran.F90:84
+ * (p3dfft_btran_c2r_many_w), build/ftran.F90:321 (p3dfft_ftran_cheby_many_w):
+ * nv variables, variable j at j*dim_in / j*dim_out elements of the array type. */
+void p3dfft_ftran_r2c_many(p3dfft_real *A, int *dim_in, p3dfft_real *B, int *dim_out, int *nv, unsigned char *op);
+void p3dfft_btran_c2r_many(p3dfft_real *A, int *dim_in, p3dfft_real *B, int *dim_out, int *nv, unsigned char *op);
+void p3dfft_cheby_many(p3dfft_real *A, int *dim_in, p3dfft_real *B, int *dim_out, int *nv, p3dfft_real *Lz);
+
 /* build/module.F90:301 (p3dfft_clean_w) */
 void p3dfft_clean(void);
 
@@ -76,6 +83,18 @@ static inline void Cp3dfft_btran_c2r(p3dfft_real *A, p3dfft_real *B, unsigned ch
   p3dfft_btran_c2r(A, B, op);
 }
 static inline void Cp3dfft_cheby(p3dfft_real *A, p3dfft_real *B, p3dfft_real Lz) { p3dfft_cheby(A, B, &Lz); }
+static inline void Cp3dfft_ftran_r2c_many(p3dfft_real *A, int dim_in, p3dfft_real *B, int dim_out, int nv,
+                                          unsigned char *op) {
+  p3dfft_ftran_r2c_many(A, &dim_in, B, &dim_out, &nv, op);
+}
+static inline void Cp3dfft_btran_c2r_many(p3dfft_real *A, int dim_in, p3dfft_real *B, int dim_out, int nv,
+                                          unsigned char *op) {
+  p3dfft_btran_c2r_many(A, &dim_in, B, &dim_out, &nv, op);
+}
+static inline void Cp3dfft_cheby_many(p3dfft_real *A, int dim_in, p3dfft_real *B, int dim_out, int nv,
+                                      p3dfft_real Lz) {
+  p3dfft_cheby_many(A, &dim_in, B, &dim_out, &nv, &Lz);
+}
 
 /* ---- extensions (no counterpart in the reference) -------------------- */
 
@@ -86,6 +105,13 @@ static inline void Cp3dfft_cheby(p3dfft_real *A, p3dfft_real *B, p3dfft_real Lz)
 int p3dfft_ftran_r2c_dev(const p3dfft_real *in, p3dfft_real *out, const unsigned char *op, void *stream);
 int p3dfft_btran_c2r_dev(p3dfft_real *in, p3dfft_real *out, const unsigned char *op, void *stream);
 int p3dfft_cheby_dev(const p3dfft_real *in, p3dfft_real *out, p3dfft_real Lz, void *stream);
+/* device-pointer _many variants; dims by value */
+int p3dfft_ftran_r2c_many_dev(const p3dfft_real *in, long long dim_in, p3dfft_real *out, long long dim_out, int nv,
+                              const unsigned char *op, void *stream);
+int p3dfft_btran_c2r_many_dev(p3dfft_real *in, long long dim_in, p3dfft_real *out, long long dim_out, int nv,
+                              const unsigned char *op, void *stream);
+int p3dfft_cheby_many_dev(const p3dfft_real *in, long long dim_in, p3dfft_real *out, long long dim_out, int nv,
+                          p3dfft_real Lz, void *stream);
 
 /* ---- share mode: one rank of a P1 x P2 grid (no counterpart entry points in
  * the reference; they split build/ftran.F90:489-780 and build/btran.F90:396-679
@@ -107,6 +133,22 @@ int p3dfft_share_ftran_z(p3dfft_real *recv2, p3dfft_real *out, const unsigned ch
 int p3dfft_share_btran_z(p3dfft_real *in, p3dfft_real *send, const unsigned char *op, void *stream);
 int p3dfft_share_btran_y(const p3dfft_real *recv, p3dfft_real *send, void *stream);
 int p3dfft_share_btran_x(const p3dfft_real *recv, p3dfft_real *out, void *stream);
+
+/* ---- spectral Poisson solves on resident fields (served / batched path;
+ * no counterpart in the reference): r2c, multiply by -1/|k|^2 (k=0 -> 0) and
+ * 1/(nx ny nz), c2r, in place on `field` (real (nx,ny,nz) in HBM), using
+ * `work` (>= p3dfft_poisson_work_bytes) on `stream`.  Independent of
+ * p3dfft_setup; any number of solves may be in flight on different streams. */
+size_t p3dfft_poisson_work_bytes(int nx, int ny, int nz);
+int p3dfft_poisson_solve(int nx, int ny, int nz, p3dfft_real *field, void *work, size_t work_bytes, void *stream);
+/* build the plan and twiddle table of a transform length ahead of time */
+int p3dfft_amd_prepare_length(int n);
+/* i.i.d. N(0,1) values from a counter-based generator keyed by seed */
+int p3dfft_amd_fill_normal(p3dfft_real *p, long long n, unsigned long long seed, void *stream);
+void *p3dfft_amd_stream_create(void);
+int p3dfft_amd_stream_destroy(void *stream);
+int p3dfft_amd_stream_synchronize(void *stream);
+int p3dfft_amd_event_query(void *event);
 
 /* Select the (Z,Y,X) complex layout for the next p3dfft_setup: the
  * reference's compile-time STRIDE1 option as a run-time switch. */

@@ -64,6 +64,17 @@ def load(precision=8):
         getattr(L, n).restype = ctypes.c_int
     L.p3dfft_cheby_dev.argtypes = [_vp, _vp, rt, _vp]
     L.p3dfft_cheby_dev.restype = ctypes.c_int
+    for n in ("p3dfft_ftran_r2c_many", "p3dfft_btran_c2r_many"):
+        getattr(L, n).argtypes = [_vp, _ip, _vp, _ip, _ip, ctypes.c_char_p]
+        getattr(L, n).restype = None
+    L.p3dfft_cheby_many.argtypes = [_vp, _ip, _vp, _ip, _ip, rp]
+    L.p3dfft_cheby_many.restype = None
+    _ll = ctypes.c_longlong
+    for n in ("p3dfft_ftran_r2c_many_dev", "p3dfft_btran_c2r_many_dev"):
+        getattr(L, n).argtypes = [_vp, _ll, _vp, _ll, ctypes.c_int, ctypes.c_char_p, _vp]
+        getattr(L, n).restype = ctypes.c_int
+    L.p3dfft_cheby_many_dev.argtypes = [_vp, _ll, _vp, _ll, ctypes.c_int, rt, _vp]
+    L.p3dfft_cheby_many_dev.restype = ctypes.c_int
     L.p3dfft_amd_set_stride1.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_error_mode.argtypes = [ctypes.c_int]
     L.p3dfft_amd_last_error.restype = ctypes.c_char_p
@@ -213,6 +224,46 @@ class P3DFFT:
         self.lib.p3dfft_cheby(a.ctypes.data, out.ctypes.data, ctypes.byref(lz))
         self._raise_if_error()
         return out
+
+    # -- multivariable (build/ftran.F90:104, build/btran.F90:104) ----------
+    def ftran_r2c_many(self, a, dim_in, dim_out, nv, op="fft"):
+        """a: flat real array of nv*dim_in; returns a flat complex array of nv*dim_out."""
+        a = np.ascontiguousarray(a, dtype=self.rdtype).reshape(-1)
+        assert a.size >= nv * dim_in
+        out = np.zeros(nv * dim_out, dtype=self.cdtype)
+        ci, co, cn = ctypes.c_int(dim_in), ctypes.c_int(dim_out), ctypes.c_int(nv)
+        self.lib.p3dfft_ftran_r2c_many(a.ctypes.data, ctypes.byref(ci), out.ctypes.data, ctypes.byref(co),
+                                       ctypes.byref(cn), _op(op))
+        self._raise_if_error()
+        return out
+
+    def btran_c2r_many(self, s, dim_in, dim_out, nv, op="tff"):
+        s = np.ascontiguousarray(s, dtype=self.cdtype).reshape(-1)
+        assert s.size >= nv * dim_in
+        out = np.zeros(nv * dim_out, dtype=self.rdtype)
+        ci, co, cn = ctypes.c_int(dim_in), ctypes.c_int(dim_out), ctypes.c_int(nv)
+        self.lib.p3dfft_btran_c2r_many(s.ctypes.data, ctypes.byref(ci), out.ctypes.data, ctypes.byref(co),
+                                       ctypes.byref(cn), _op(op))
+        self._raise_if_error()
+        return out
+
+    def cheby_many(self, a, dim_in, dim_out, nv, Lz):
+        a = np.ascontiguousarray(a, dtype=self.rdtype).reshape(-1)
+        out = np.zeros(nv * dim_out, dtype=self.cdtype)
+        ci, co, cn = ctypes.c_int(dim_in), ctypes.c_int(dim_out), ctypes.c_int(nv)
+        lz = (ctypes.c_double if self.precision == 8 else ctypes.c_float)(Lz)
+        self.lib.p3dfft_cheby_many(a.ctypes.data, ctypes.byref(ci), out.ctypes.data, ctypes.byref(co),
+                                   ctypes.byref(cn), ctypes.byref(lz))
+        self._raise_if_error()
+        return out
+
+    def ftran_r2c_many_dev(self, in_ptr, dim_in, out_ptr, dim_out, nv, op="fft", stream=None):
+        if self.lib.p3dfft_ftran_r2c_many_dev(in_ptr, dim_in, out_ptr, dim_out, nv, _op(op), stream):
+            self._raise_if_error(force=True)
+
+    def btran_c2r_many_dev(self, in_ptr, dim_in, out_ptr, dim_out, nv, op="tff", stream=None):
+        if self.lib.p3dfft_btran_c2r_many_dev(in_ptr, dim_in, out_ptr, dim_out, nv, _op(op), stream):
+            self._raise_if_error(force=True)
 
     def clean(self):
         self.lib.p3dfft_clean()

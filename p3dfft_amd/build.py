@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "bin", "amdflang")
 
-SOURCES = ["engine.hip", "share.hip", "api.cpp", "api_share.cpp"]
+SOURCES = ["engine.hip", "share.hip", "api.cpp", "api_share.cpp", "api_poisson.cpp"]
 # packed-FP32 instructions are disabled for all device code
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -68,8 +68,11 @@ def build_fortran():
         os.makedirs(md, exist_ok=True)
         obj = os.path.join(md, "p3dfft_mod.o")
         _run([FLANG, "-cpp", "-O2", "-fPIC"] + defs + ["-module-dir", md, "-c", src, "-o", obj])
+        wobj = os.path.join(md, "wrap.o")
+        _run([FLANG, "-cpp", "-O2", "-fPIC"] + defs + ["-I", md, "-module-dir", md, "-c",
+              os.path.join(HERE, "fortran", "wrap.f90"), "-o", wobj])
         base = "p3dfft_amd" if not defs else "p3dfft_amd_sp"
-        _run([FLANG, "-shared", "-fPIC", "-o", os.path.join(LIB, lib), obj,
+        _run([FLANG, "-shared", "-fPIC", "-o", os.path.join(LIB, lib), obj, wobj,
               "-L" + LIB, "-l" + base, "-Wl,-rpath,$ORIGIN"])
         outs.append(os.path.join(LIB, lib))
         # the Fortran sample driver (examples/driver_sine.f90) against this module

@@ -71,19 +71,19 @@ void timers_resolve() {
   g.pend.clear();
 }
 
-void tstart(int slot, hipStream_t s) {
-  if (!g.timing) return;
+void tstart(State& S, int slot, hipStream_t s) {
+  if (!S.timing) return;
   hipEvent_t a = ev_get();
   check_hip(hipEventRecord(a, s), "hipEventRecord");
-  g.open[slot - 1] = a;
+  S.open[slot - 1] = a;
 }
 
-void tstop(int slot, hipStream_t s) {
-  if (!g.timing) return;
+void tstop(State& S, int slot, hipStream_t s) {
+  if (!S.timing) return;
   hipEvent_t b = ev_get();
   check_hip(hipEventRecord(b, s), "hipEventRecord");
-  g.pend.push_back(Pending{slot, g.open[slot - 1], b});
-  if (g.pend.size() > 8192) timers_resolve();
+  S.pend.push_back(Pending{slot, S.open[slot - 1], b});
+  if (S.pend.size() > 8192) timers_resolve();
 }
 
 // ---------------------------------------------------------------------------
@@ -183,22 +183,22 @@ static int zkind(char c, bool* none) {
 
 // z-planes per X/Y chunk: the chunk intermediate is sized to stay resident in
 // the last-level cache; small problems run as one chunk
-int chunk_planes() {
-  const size_t plane = (size_t)g.nxhpc * g.ny * sizeof(cplx_t);
-  if (g.chunk_bytes == 0 || plane * (size_t)g.nz <= 2 * g.chunk_bytes) return g.nz;
-  size_t zc = g.chunk_bytes / plane;
+int chunk_planes(State& S) {
+  const size_t plane = (size_t)S.nxhpc * S.ny * sizeof(cplx_t);
+  if (S.chunk_bytes == 0 || plane * (size_t)S.nz <= 2 * S.chunk_bytes) return S.nz;
+  size_t zc = S.chunk_bytes / plane;
   if (zc < 1) zc = 1;
-  return (int)std::min<size_t>(zc, (size_t)g.nz);
+  return (int)std::min<size_t>(zc, (size_t)S.nz);
 }
 
 // ---------------------------------------------------------------------------
 // forward: X r2c -> Y c2c (per z-plane) -> Z op, build/ftran.F90:489-780
 // ---------------------------------------------------------------------------
-bool forward_dev(const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s) {
+bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s) {
   bool none;
   const int kind = zkind((char)op[2], &none);
   if (kind < 0) return fatal(std::string("Unknown transform type: ") + (char)op[2]);
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nxhpc = g.nxhpc, nyc = g.nyc, nzc = g.nzc;
+  const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
   const int dny = ny - nyc, dnz = nz - nzc;
   const long long nxy = (long long)nxhpc * nyc;
   // X r2c then Y c2c, z-plane chunk by chunk: the (nxhpc, ny, zc) intermediate
@@ -206,8 +206,8 @@ bool forward_dev(const real_t* in, cplx_t* out, const unsigned char* op, hipStre
   // so it is served from the last-level cache instead of HBM.
   // seg_copy_x and seg_copy_y (keep y < nycph, then y + dny) are folded into
   // the stores.
-  const bool direct = (dnz == 0 && !g.stride1);
-  cplx_t* ydst = direct ? out : g.w2;
+  const bool direct = (dnz == 0 && !S.stride1);
+  cplx_t* ydst = direct ? out : S.w2;
   // a complex output plane is larger than a real input plane: when the output
   // overlaps the input (in-place use) a chunk would overwrite input planes of
   // later chunks, so such calls run as one chunk
@@ -215,30 +215,30 @@ bool forward_dev(const real_t* in, cplx_t* out, const unsigned char* op, hipStre
   const char* ob = (const char*)ydst;
   const bool alias = ob < ib + (size_t)nx * ny * nz * sizeof(real_t) &&
                      ib < ob + (size_t)nxhpc * nyc * nz * sizeof(cplx_t);
-  const int zc = alias ? nz : chunk_planes();
+  const int zc = alias ? nz : chunk_planes(S);
   for (int z0 = 0; z0 < nz; z0 += zc) {
     const int nzk = std::min(zc, nz - z0);
-    tstart(5, s);
-    launch_x_r2c(in + (size_t)z0 * ny * nx, g.w1, (long long)ny * nzk, nx, nxhpc, nx, nxhpc, s);
-    tstop(5, s);
+    tstart(S, 5, s);
+    launch_x_r2c(in + (size_t)z0 * ny * nx, S.w1, (long long)ny * nzk, nx, nxhpc, nx, nxhpc, s);
+    tstop(S, 5, s);
     ColSpec y{};
-    y.in = g.w1; y.out = ydst + (size_t)z0 * nxy;
+    y.in = S.w1; y.out = ydst + (size_t)z0 * nxy;
     y.si_in = 1; y.sl_in = nxhpc; y.so_in = (long long)nxhpc * ny;
     y.si_out = 1; y.sl_out = nxhpc; y.so_out = nxy;
     y.n_inner = nxhpc; y.n_outer = nzk; y.n = ny; y.kind = COL_C2C; y.fwd = true;
     y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
-    y.nst = nyc; y.st_lo = g.nycph; y.st_shift = dny;
-    tstart(7, s);
+    y.nst = nyc; y.st_lo = S.nycph; y.st_shift = dny;
+    tstart(S, 7, s);
     launch_col(y, s);
-    tstop(7, s);
+    tstop(S, 7, s);
   }
   // Z op; seg_copy_z (keep z < nzcph, then z + dnz) and the STRIDE1 layout
   // folded into the store
-  tstart(8, s);
+  tstart(S, 8, s);
   if (!(direct && none)) {
     ColSpec z{};
     z.in = ydst; z.out = out;
-    if (g.stride1) {
+    if (S.stride1) {
       z.n_inner = nxhpc; z.n_outer = nyc;
       z.si_in = 1; z.sl_in = nxy; z.so_in = nxhpc;
       z.si_out = (long long)nzc * nyc; z.sl_out = 1; z.so_out = nzc;
@@ -249,26 +249,26 @@ bool forward_dev(const real_t* in, cplx_t* out, const unsigned char* op, hipStre
     }
     z.n = nz; z.kind = kind; z.fwd = true;
     z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
-    z.nst = nzc; z.st_lo = g.nzcph; z.st_shift = dnz;
+    z.nst = nzc; z.st_lo = S.nzcph; z.st_shift = dnz;
     launch_col(z, s);
   }
-  tstop(8, s);
+  tstop(S, 8, s);
   return true;
 }
 
 // ---------------------------------------------------------------------------
 // backward: Z op -> Y c2c -> X c2r, build/btran.F90:396-679
 // ---------------------------------------------------------------------------
-bool backward_dev(cplx_t* in, real_t* out, const unsigned char* op, hipStream_t s) {
+bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hipStream_t s) {
   bool none;
   const int kind = zkind((char)op[0], &none);
   if (kind < 0) return fatal(std::string(" Unknown transform type: ") + (char)op[0]);
-  const int nx = g.nx, ny = g.ny, nz = g.nz, nxhpc = g.nxhpc, nyc = g.nyc, nzc = g.nzc;
+  const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
   const int dny = ny - nyc, dnz = nz - nzc;
   const long long nxy = (long long)nxhpc * nyc;
   cplx_t* zdst;
-  tstart(9, s);
-  if (!g.stride1 && g.ow && nz == nzc) {
+  tstart(S, 9, s);
+  if (!S.stride1 && S.ow && nz == nzc) {
     // Z op in place on the caller's array (btran.F90:525-550): the input
     // spectrum is overwritten, as in the reference
     if (!none) {
@@ -286,8 +286,8 @@ bool backward_dev(cplx_t* in, real_t* out, const unsigned char* op, hipStream_t 
     // seg_copy_z + seg_zero_z folded into the load (btran.F90:556-558;
     // STRIDE1: reorder.F90:208-245 keeps z >= nzcph+dnz)
     ColSpec z{};
-    z.in = in; z.out = g.w2;
-    if (g.stride1) {
+    z.in = in; z.out = S.w2;
+    if (S.stride1) {
       z.n_inner = nxhpc; z.n_outer = nyc;
       z.si_in = (long long)nzc * nyc; z.sl_in = 1; z.so_in = nzc;
       z.si_out = 1; z.sl_out = nxy; z.so_out = nxhpc;
@@ -297,43 +297,229 @@ bool backward_dev(cplx_t* in, real_t* out, const unsigned char* op, hipStream_t 
       z.si_out = 1; z.sl_out = nxy; z.so_out = 0;
     }
     z.n = nz; z.kind = kind; z.fwd = false;
-    z.nld = nz; z.ld_lo = g.nzcph; z.ld_hi = g.stride1 ? g.nzcph + dnz : nz - g.nzcph; z.ld_shift = dnz;
+    z.nld = nz; z.ld_lo = S.nzcph; z.ld_hi = S.stride1 ? S.nzcph + dnz : nz - S.nzcph; z.ld_shift = dnz;
     z.nst = nz; z.st_lo = nz; z.st_shift = 0;
     launch_col(z, s);
-    zdst = g.w2;
+    zdst = S.w2;
   }
-  tstop(9, s);
+  tstop(S, 9, s);
   // Y c2c backward then X c2r, chunk by chunk over z-planes (the chunk's
   // intermediate stays in the last-level cache); seg_copy_y + seg_zero_y and
   // seg_copy_x + seg_zero_x (x > nxhpc) are folded into the loads
-  const int zc = chunk_planes();
+  const int zc = chunk_planes(S);
   for (int z0 = 0; z0 < nz; z0 += zc) {
     const int nzk = std::min(zc, nz - z0);
     ColSpec y{};
-    y.in = zdst + (size_t)z0 * nxy; y.out = g.w1;
+    y.in = zdst + (size_t)z0 * nxy; y.out = S.w1;
     y.si_in = 1; y.sl_in = nxhpc; y.so_in = nxy;
     y.si_out = 1; y.sl_out = nxhpc; y.so_out = (long long)nxhpc * ny;
     y.n_inner = nxhpc; y.n_outer = nzk; y.n = ny; y.kind = COL_C2C; y.fwd = false;
-    y.nld = ny; y.ld_lo = g.nycph; y.ld_hi = g.stride1 ? g.nycph + dny : ny - g.nycph; y.ld_shift = dny;
+    y.nld = ny; y.ld_lo = S.nycph; y.ld_hi = S.stride1 ? S.nycph + dny : ny - S.nycph; y.ld_shift = dny;
     y.nst = ny; y.st_lo = ny; y.st_shift = 0;
-    tstart(10, s);
+    tstart(S, 10, s);
     launch_col(y, s);
-    tstop(10, s);
-    tstart(12, s);
-    launch_x_c2r(g.w1, out + (size_t)z0 * ny * nx, (long long)ny * nzk, nxhpc, nx, nx, nxhpc, s);
-    tstop(12, s);
+    tstop(S, 10, s);
+    tstart(S, 12, s);
+    launch_x_c2r(S.w1, out + (size_t)z0 * ny * nx, (long long)ny * nzk, nxhpc, nx, nx, nxhpc, s);
+    tstop(S, 12, s);
   }
   return true;
 }
 
-bool cheby_dev(const real_t* in, cplx_t* out, double Lz, hipStream_t s) {
+bool cheby_dev(State& S, const real_t* in, cplx_t* out, double Lz, hipStream_t s) {
   static const unsigned char ffc[4] = {'f', 'f', 'c', 0};
-  if (g.nzc < 2) return fatal("p3dfft_cheby: nzc must be at least 2");
-  if (!forward_dev(in, out, ffc, s)) return false;
-  const double scale = 1.0 / ((double)g.nx * (double)g.ny * (double)(g.nzc - 1));
-  const long long ncol = (long long)g.nxhpc * g.nyc;
-  if (g.stride1) launch_cheby(out, ncol, g.nzc, g.nzc, 1, scale, 4.0 / Lz, s);
-  else launch_cheby(out, ncol, g.nzc, 1, ncol, scale, 4.0 / Lz, s);
+  if (S.nzc < 2) return fatal("p3dfft_cheby: nzc must be at least 2");
+  if (!forward_dev(S, in, out, ffc, s)) return false;
+  const double scale = 1.0 / ((double)S.nx * (double)S.ny * (double)(S.nzc - 1));
+  const long long ncol = (long long)S.nxhpc * S.nyc;
+  if (S.stride1) launch_cheby(out, ncol, S.nzc, S.nzc, 1, scale, 4.0 / Lz, s);
+  else launch_cheby(out, ncol, S.nzc, 1, ncol, scale, 4.0 / Lz, s);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// multivariable transforms (build/ftran.F90:104-317, build/btran.F90:104-304):
+// nv variables, variable j of the input at j*dim_in, of the output at
+// j*dim_out (elements of the array's type).  Each stage covers all variables
+// in one launch where the layouts allow it.
+// ---------------------------------------------------------------------------
+void ensure_many(State& S, int nv) {
+  if (nv <= S.nv_alloc) return;
+  if (S.w1) (void)hipFree(S.w1);
+  if (S.w2) (void)hipFree(S.w2);
+  S.w1 = S.w2 = nullptr;
+  S.nv_alloc = 0;
+  check_hip(hipMalloc(&S.w1, (size_t)S.nxhpc * S.ny * S.nz * nv * sizeof(cplx_t)), "hipMalloc(work)");
+  check_hip(hipMalloc(&S.w2, (size_t)S.nxhpc * S.nyc * S.nz * nv * sizeof(cplx_t)), "hipMalloc(work)");
+  S.nv_alloc = nv;
+}
+
+bool forward_many_dev(State& S, const real_t* in, long long dim_in, cplx_t* out, long long dim_out, int nv,
+                      const unsigned char* op, hipStream_t s) {
+  bool none;
+  const int kind = zkind((char)op[2], &none);
+  if (kind < 0) return fatal(std::string("Unknown transform type: ") + (char)op[2]);
+  if (nv <= 0) return true;
+  ensure_many(S, nv);
+  const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
+  const long long nxy = (long long)nxhpc * nyc;
+  const long long nreal = (long long)nx * ny * nz;
+  // X r2c of every variable into w1 [nv][nz][ny][nxhpc]
+  tstart(S, 5, s);
+  if (dim_in == nreal) {
+    launch_x_r2c(in, S.w1, (long long)ny * nz * nv, nx, nxhpc, nx, nxhpc, s);
+  } else {
+    for (int j = 0; j < nv; j++)
+      launch_x_r2c(in + (size_t)j * dim_in, S.w1 + (size_t)j * nxhpc * ny * nz, (long long)ny * nz, nx, nxhpc,
+                   nx, nxhpc, s);
+  }
+  tstop(S, 5, s);
+  // Y c2c over all nz*nv planes into w2 [nv][nz][nyc][nxhpc]
+  ColSpec y{};
+  y.in = S.w1; y.out = S.w2;
+  y.si_in = 1; y.sl_in = nxhpc; y.so_in = (long long)nxhpc * ny;
+  y.si_out = 1; y.sl_out = nxhpc; y.so_out = nxy;
+  y.n_inner = nxhpc; y.n_outer = nz * nv; y.n = ny; y.kind = COL_C2C; y.fwd = true;
+  y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
+  y.nst = nyc; y.st_lo = S.nycph; y.st_shift = ny - nyc;
+  tstart(S, 7, s);
+  launch_col(y, s);
+  tstop(S, 7, s);
+  // Z op from w2 into every output variable
+  tstart(S, 8, s);
+  ColSpec z{};
+  z.n = nz; z.kind = kind; z.fwd = true;
+  z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
+  z.nst = nzc; z.st_lo = S.nzcph; z.st_shift = nz - nzc;
+  if (!S.stride1) {
+    z.in = S.w2; z.out = out;
+    z.n_inner = (int)nxy; z.n_outer = nv;
+    z.si_in = 1; z.sl_in = nxy; z.so_in = nxy * nz;
+    z.si_out = 1; z.sl_out = nxy; z.so_out = dim_out;
+    launch_col(z, s);
+  } else {
+    for (int j = 0; j < nv; j++) {
+      z.in = S.w2 + (size_t)j * nxy * nz; z.out = out + (size_t)j * dim_out;
+      z.n_inner = nxhpc; z.n_outer = nyc;
+      z.si_in = 1; z.sl_in = nxy; z.so_in = nxhpc;
+      z.si_out = (long long)nzc * nyc; z.sl_out = 1; z.so_out = nzc;
+      launch_col(z, s);
+    }
+  }
+  tstop(S, 8, s);
+  return true;
+}
+
+bool backward_many_dev(State& S, cplx_t* in, long long dim_in, real_t* out, long long dim_out, int nv,
+                       const unsigned char* op, hipStream_t s) {
+  bool none;
+  const int kind = zkind((char)op[0], &none);
+  if (kind < 0) return fatal(std::string(" Unknown transform type: ") + (char)op[0]);
+  if (nv <= 0) return true;
+  ensure_many(S, nv);
+  const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
+  const int dny = ny - nyc, dnz = nz - nzc;
+  const long long nxy = (long long)nxhpc * nyc;
+  // Z op: in place on the caller's arrays (overwrite, nz == nzc, XYZ layout)
+  // or from the caller's arrays into w2 [nv][nz][nyc][nxhpc]
+  const bool inplace = !S.stride1 && S.ow && nz == nzc;
+  tstart(S, 9, s);
+  ColSpec z{};
+  z.n = nz; z.kind = kind; z.fwd = false;
+  z.nst = nz; z.st_lo = nz; z.st_shift = 0;
+  if (inplace) {
+    if (!none) {
+      z.in = in; z.out = in;
+      z.n_inner = (int)nxy; z.n_outer = nv;
+      z.si_in = 1; z.sl_in = nxy; z.so_in = dim_in;
+      z.si_out = 1; z.sl_out = nxy; z.so_out = dim_in;
+      z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
+      launch_col(z, s);
+    }
+  } else {
+    z.nld = nz; z.ld_lo = S.nzcph; z.ld_hi = S.stride1 ? S.nzcph + dnz : nz - S.nzcph; z.ld_shift = dnz;
+    if (!S.stride1) {
+      z.in = in; z.out = S.w2;
+      z.n_inner = (int)nxy; z.n_outer = nv;
+      z.si_in = 1; z.sl_in = nxy; z.so_in = dim_in;
+      z.si_out = 1; z.sl_out = nxy; z.so_out = nxy * nz;
+      launch_col(z, s);
+    } else {
+      for (int j = 0; j < nv; j++) {
+        z.in = in + (size_t)j * dim_in; z.out = S.w2 + (size_t)j * nxy * nz;
+        z.n_inner = nxhpc; z.n_outer = nyc;
+        z.si_in = (long long)nzc * nyc; z.sl_in = 1; z.so_in = nzc;
+        z.si_out = 1; z.sl_out = nxy; z.so_out = nxhpc;
+        launch_col(z, s);
+      }
+    }
+  }
+  tstop(S, 9, s);
+  // Y c2c backward into w1 [nv][nz][ny][nxhpc]
+  ColSpec y{};
+  y.out = S.w1;
+  y.si_in = 1; y.sl_in = nxhpc; y.so_in = nxy;
+  y.si_out = 1; y.sl_out = nxhpc; y.so_out = (long long)nxhpc * ny;
+  y.n_inner = nxhpc; y.n = ny; y.kind = COL_C2C; y.fwd = false;
+  y.nld = ny; y.ld_lo = S.nycph; y.ld_hi = S.stride1 ? S.nycph + dny : ny - S.nycph; y.ld_shift = dny;
+  y.nst = ny; y.st_lo = ny; y.st_shift = 0;
+  tstart(S, 10, s);
+  if (!inplace || dim_in == nxy * nz) {
+    y.in = inplace ? in : S.w2;
+    y.n_outer = nz * nv;
+    launch_col(y, s);
+  } else {
+    for (int j = 0; j < nv; j++) {
+      y.in = in + (size_t)j * dim_in;
+      y.out = S.w1 + (size_t)j * nxhpc * ny * nz;
+      y.n_outer = nz;
+      launch_col(y, s);
+    }
+  }
+  tstop(S, 10, s);
+  // X c2r of every variable
+  tstart(S, 12, s);
+  if (dim_out == (long long)nx * ny * nz) {
+    launch_x_c2r(S.w1, out, (long long)ny * nz * nv, nxhpc, nx, nx, nxhpc, s);
+  } else {
+    for (int j = 0; j < nv; j++)
+      launch_x_c2r(S.w1 + (size_t)j * nxhpc * ny * nz, out + (size_t)j * dim_out, (long long)ny * nz, nxhpc, nx,
+                   nx, nxhpc, s);
+  }
+  tstop(S, 12, s);
+  return true;
+}
+
+bool cheby_many_dev(State& S, const real_t* in, long long dim_in, cplx_t* out, long long dim_out, int nv, double Lz,
+                    hipStream_t s) {
+  static const unsigned char ffc[4] = {'f', 'f', 'c', 0};
+  if (S.nzc < 2) return fatal("p3dfft_cheby: nzc must be at least 2");
+  if (!forward_many_dev(S, in, dim_in, out, dim_out, nv, ffc, s)) return false;
+  const double scale = 1.0 / ((double)S.nx * (double)S.ny * (double)(S.nzc - 1));
+  const long long ncol = (long long)S.nxhpc * S.nyc;
+  for (int j = 0; j < nv; j++) {
+    cplx_t* o = out + (size_t)j * dim_out;
+    if (S.stride1) launch_cheby(o, ncol, S.nzc, S.nzc, 1, scale, 4.0 / Lz, s);
+    else launch_cheby(o, ncol, S.nzc, 1, ncol, scale, 4.0 / Lz, s);
+  }
+  return true;
+}
+
+// The reference prints these and continues (build/ftran.F90:123-129).
+static bool check_many_dims(long long dim_in, long long dim_out, bool fwd) {
+  const long long nr = (long long)g.nx * g.jisize * g.kjsize;
+  const long long nc = (long long)g.nzc * g.jjsize * g.iisize;
+  const long long need_in = fwd ? nr : nc, need_out = fwd ? nc : nr;
+  if (dim_in < need_in) {
+    notice(std::string(fwd ? " ftran" : " btran") + " error: input array dimensions are too low: " +
+           std::to_string(dim_in) + " while expecting " + std::to_string(need_in));
+    return false;
+  }
+  if (dim_out < need_out) {
+    notice(std::string(fwd ? " ftran" : " btran") + " error: output array dimensions are too low: " +
+           std::to_string(dim_out) + " while expecting " + std::to_string(need_out));
+    return false;
+  }
   return true;
 }
 
@@ -363,6 +549,7 @@ void release_state() {
   if (g.stage_a) (void)hipFree(g.stage_a);
   if (g.stage_b) (void)hipFree(g.stage_b);
   g.w1 = g.w2 = nullptr;
+  g.nv_alloc = 0;
   g.stage_a = g.stage_b = nullptr;
   g.stage_bytes = 0;
   g.set = false;
@@ -421,6 +608,7 @@ void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int
   }
   check_hip(hipMalloc(&g.w1, (size_t)g.nxhpc * g.ny * g.nz * sizeof(cplx_t)), "hipMalloc(work)");
   check_hip(hipMalloc(&g.w2, (size_t)g.nxhpc * g.nyc * g.nz * sizeof(cplx_t)), "hipMalloc(work)");
+  g.nv_alloc = 1;
   // build the 1D plans (twiddles are generated on the GPU here, once)
   get_len(g.nx);
   get_len(g.ny);
@@ -460,7 +648,7 @@ void p3dfft_ftran_r2c(real_t* A, real_t* B, unsigned char* op) {
   GUARD_BEGIN
   ensure_staging();
   check_hip(hipMemcpy(g.stage_a, A, real_elems() * sizeof(real_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
-  if (!forward_dev((const real_t*)g.stage_a, (cplx_t*)g.stage_b, op, g.stream)) return;
+  if (!forward_dev(g, (const real_t*)g.stage_a, (cplx_t*)g.stage_b, op, g.stream)) return;
   check_hip(hipMemcpy(B, g.stage_b, spec_elems() * sizeof(cplx_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
   GUARD_END
 }
@@ -471,7 +659,7 @@ void p3dfft_btran_c2r(real_t* A, real_t* B, unsigned char* op) {
   GUARD_BEGIN
   ensure_staging();
   check_hip(hipMemcpy(g.stage_a, A, spec_elems() * sizeof(cplx_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
-  if (!backward_dev((cplx_t*)g.stage_a, (real_t*)g.stage_b, op, g.stream)) return;
+  if (!backward_dev(g, (cplx_t*)g.stage_a, (real_t*)g.stage_b, op, g.stream)) return;
   check_hip(hipMemcpy(B, g.stage_b, real_elems() * sizeof(real_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
   GUARD_END
 }
@@ -482,8 +670,64 @@ void p3dfft_cheby(real_t* A, real_t* B, real_t* Lz) {
   GUARD_BEGIN
   ensure_staging();
   check_hip(hipMemcpy(g.stage_a, A, real_elems() * sizeof(real_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
-  if (!cheby_dev((const real_t*)g.stage_a, (cplx_t*)g.stage_b, (double)*Lz, g.stream)) return;
+  if (!cheby_dev(g, (const real_t*)g.stage_a, (cplx_t*)g.stage_b, (double)*Lz, g.stream)) return;
   check_hip(hipMemcpy(B, g.stage_b, spec_elems() * sizeof(cplx_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  GUARD_END
+}
+
+// staging for nv variables of the host-array _many entry points
+static void ensure_staging_many(size_t in_bytes, size_t out_bytes) {
+  const size_t bytes = std::max(in_bytes, out_bytes);
+  if (g.stage_bytes >= bytes) return;
+  if (g.stage_a) (void)hipFree(g.stage_a);
+  if (g.stage_b) (void)hipFree(g.stage_b);
+  g.stage_a = g.stage_b = nullptr;
+  g.stage_bytes = 0;
+  check_hip(hipMalloc(&g.stage_a, bytes), "hipMalloc(staging)");
+  check_hip(hipMalloc(&g.stage_b, bytes), "hipMalloc(staging)");
+  g.stage_bytes = bytes;
+}
+
+// build/ftran.F90:84-100 (BIND(C) p3dfft_ftran_r2c_many) and 104-317
+void p3dfft_ftran_r2c_many(real_t* A, int* dim_in, real_t* B, int* dim_out, int* nv, unsigned char* op) {
+  if (!require_full()) return;
+  GUARD_BEGIN
+  // the reference prints and continues; continuing here would index past the
+  // staging buffers, so the call returns
+  if (!check_many_dims(*dim_in, *dim_out, true)) return;
+  const size_t ib = (size_t)*dim_in * *nv * sizeof(real_t), ob = (size_t)*dim_out * *nv * sizeof(cplx_t);
+  ensure_staging_many(ib, ob);
+  check_hip(hipMemcpy(g.stage_a, A, ib, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  if (!forward_many_dev(g, (const real_t*)g.stage_a, *dim_in, (cplx_t*)g.stage_b, *dim_out, *nv, op, g.stream)) return;
+  check_hip(hipMemcpy(B, g.stage_b, ob, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  GUARD_END
+}
+
+// build/btran.F90:84-100 and 104-304
+void p3dfft_btran_c2r_many(real_t* A, int* dim_in, real_t* B, int* dim_out, int* nv, unsigned char* op) {
+  if (!require_full()) return;
+  GUARD_BEGIN
+  if (!check_many_dims(*dim_in, *dim_out, false)) return;
+  const size_t ib = (size_t)*dim_in * *nv * sizeof(cplx_t), ob = (size_t)*dim_out * *nv * sizeof(real_t);
+  ensure_staging_many(ib, ob);
+  check_hip(hipMemcpy(g.stage_a, A, ib, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  if (!backward_many_dev(g, (cplx_t*)g.stage_a, *dim_in, (real_t*)g.stage_b, *dim_out, *nv, op, g.stream)) return;
+  check_hip(hipMemcpy(B, g.stage_b, ob, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  GUARD_END
+}
+
+// build/ftran.F90:321-354 (the reference re-runs the forward per variable
+// inside p3dfft_cheby; the result is the same)
+void p3dfft_cheby_many(real_t* A, int* dim_in, real_t* B, int* dim_out, int* nv, real_t* Lz) {
+  if (!require_full()) return;
+  GUARD_BEGIN
+  if (!check_many_dims(*dim_in, *dim_out, true)) return;
+  const size_t ib = (size_t)*dim_in * *nv * sizeof(real_t), ob = (size_t)*dim_out * *nv * sizeof(cplx_t);
+  ensure_staging_many(ib, ob);
+  check_hip(hipMemcpy(g.stage_a, A, ib, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  if (!cheby_many_dev(g, (const real_t*)g.stage_a, *dim_in, (cplx_t*)g.stage_b, *dim_out, *nv, (double)*Lz, g.stream))
+    return;
+  check_hip(hipMemcpy(B, g.stage_b, ob, hipMemcpyDeviceToHost), "hipMemcpy D2H");
   GUARD_END
 }
 
@@ -519,7 +763,7 @@ void set_timers(void) {
 int p3dfft_ftran_r2c_dev(const real_t* in, real_t* out, const unsigned char* op, void* stream) {
   if (!require_full()) return -1;
   GUARD_BEGIN
-  return forward_dev(in, (cplx_t*)out, op, (hipStream_t)stream) ? 0 : -1;
+  return forward_dev(g, in, (cplx_t*)out, op, (hipStream_t)stream) ? 0 : -1;
   GUARD_END
   return -1;
 }
@@ -527,7 +771,7 @@ int p3dfft_ftran_r2c_dev(const real_t* in, real_t* out, const unsigned char* op,
 int p3dfft_btran_c2r_dev(real_t* in, real_t* out, const unsigned char* op, void* stream) {
   if (!require_full()) return -1;
   GUARD_BEGIN
-  return backward_dev((cplx_t*)in, out, op, (hipStream_t)stream) ? 0 : -1;
+  return backward_dev(g, (cplx_t*)in, out, op, (hipStream_t)stream) ? 0 : -1;
   GUARD_END
   return -1;
 }
@@ -535,7 +779,37 @@ int p3dfft_btran_c2r_dev(real_t* in, real_t* out, const unsigned char* op, void*
 int p3dfft_cheby_dev(const real_t* in, real_t* out, real_t Lz, void* stream) {
   if (!require_full()) return -1;
   GUARD_BEGIN
-  return cheby_dev(in, (cplx_t*)out, (double)Lz, (hipStream_t)stream) ? 0 : -1;
+  return cheby_dev(g, in, (cplx_t*)out, (double)Lz, (hipStream_t)stream) ? 0 : -1;
+  GUARD_END
+  return -1;
+}
+
+int p3dfft_ftran_r2c_many_dev(const real_t* in, long long dim_in, real_t* out, long long dim_out, int nv,
+                              const unsigned char* op, void* stream) {
+  if (!require_full()) return -1;
+  GUARD_BEGIN
+  if (!check_many_dims(dim_in, dim_out, true)) return -1;
+  return forward_many_dev(g, in, dim_in, (cplx_t*)out, dim_out, nv, op, (hipStream_t)stream) ? 0 : -1;
+  GUARD_END
+  return -1;
+}
+
+int p3dfft_btran_c2r_many_dev(real_t* in, long long dim_in, real_t* out, long long dim_out, int nv,
+                              const unsigned char* op, void* stream) {
+  if (!require_full()) return -1;
+  GUARD_BEGIN
+  if (!check_many_dims(dim_in, dim_out, false)) return -1;
+  return backward_many_dev(g, (cplx_t*)in, dim_in, out, dim_out, nv, op, (hipStream_t)stream) ? 0 : -1;
+  GUARD_END
+  return -1;
+}
+
+int p3dfft_cheby_many_dev(const real_t* in, long long dim_in, real_t* out, long long dim_out, int nv, real_t Lz,
+                          void* stream) {
+  if (!require_full()) return -1;
+  GUARD_BEGIN
+  if (!check_many_dims(dim_in, dim_out, true)) return -1;
+  return cheby_many_dev(g, in, dim_in, (cplx_t*)out, dim_out, nv, (double)Lz, (hipStream_t)stream) ? 0 : -1;
   GUARD_END
   return -1;
 }

@@ -1,0 +1,150 @@
+// api_poisson.cpp -- spectral Poisson solves on resident HBM fields, the unit of
+// work of the served / batched path (no counterpart in the reference; it
+// composes the reference's forward and backward transforms, build/ftran.F90:
+// 489-780 and build/btran.F90:396-679, around a per-mode multiply).
+//
+// A solve of -lap(u) = -f on the periodic box of an (nx, ny, nz) grid:
+//   f_hat = r2c(f);  u_hat = -f_hat / |k|^2 / (nx ny nz), u_hat(k=0) = 0;
+//   u = c2r(u_hat)
+// in place on the field, with integer wavenumbers k = j (j <= n/2) or j - n.
+// Solves take their work area from the caller, so any number of solves of
+// any shapes can be in flight on different streams.
+#include <algorithm>
+#include <cstdio>
+#include <string>
+
+#include "engine.hpp"
+#include "p3dfft_amd_internal.hpp"
+
+using namespace p3amd;
+
+namespace p3amd {
+
+template <class T>
+__global__ void k_poisson(cpx<T>* spec, int nxhp, int ny, int nz, double scale) {
+  const long long n = (long long)nxhp * ny * nz;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = (int)(i % nxhp);
+  const long long r = i / nxhp;
+  const int y = (int)(r % ny);
+  const int z = (int)(r / ny);
+  const double ky = y <= ny / 2 ? y : y - ny;
+  const double kz = z <= nz / 2 ? z : z - nz;
+  const double k2 = (double)x * x + ky * ky + kz * kz;
+  const double f = k2 > 0.0 ? -scale / k2 : 0.0;
+  const cpx<T> v = spec[i];
+  spec[i] = mk<T>((T)(v.x * f), (T)(v.y * f));
+}
+
+// device RNG: counter-based (splitmix64 finaliser) uniform pairs, Box-Muller
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <class T>
+__global__ void k_fill_normal(T* p, long long n, uint64_t seed) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t a = mix64(seed * 0x100000001B3ull + 2 * (uint64_t)i);
+  const uint64_t b = mix64(seed * 0x100000001B3ull + 2 * (uint64_t)i + 1);
+  const double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);  // (0, 1]
+  const double u2 = (b >> 11) * (1.0 / 9007199254740992.0);          // [0, 1)
+  p[i] = (T)(sqrt(-2.0 * log(u1)) * cospi(2.0 * u2));
+}
+
+// geometry of a full, unpruned 1x1 transform of (nx, ny, nz) with overwrite
+static void poisson_geometry(State& S, int nx, int ny, int nz) {
+  S.nx = nx; S.ny = ny; S.nz = nz;
+  S.nxc = nx; S.nyc = ny; S.nzc = nz;
+  S.ow = true;
+  S.stride1 = false;
+  S.timing = false;
+  const int dims[2] = {1, 1};
+  compute_geometry(S, dims, 0, 0);
+}
+
+size_t poisson_work_bytes(int nx, int ny, int nz) {
+  // spectrum + X/Y intermediate
+  return 2 * (size_t)(nx / 2 + 1) * ny * nz * sizeof(cplx_t);
+}
+
+}  // namespace p3amd
+
+extern "C" {
+
+// Build (once) the plan and twiddle table of a transform length, so that later
+// calls of any entry point with that length launch without set-up work.
+int p3dfft_amd_prepare_length(int n) {
+  try {
+    if (n <= 0 || n > max_len()) return -1;
+    get_len(n);
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return -1;
+  }
+}
+
+// HBM work bytes one solve of this shape needs besides its field.
+size_t p3dfft_poisson_work_bytes(int nx, int ny, int nz) { return poisson_work_bytes(nx, ny, nz); }
+
+// Solve in place on `field` (real (nx, ny, nz), x fastest, in HBM) using
+// `work` (at least p3dfft_poisson_work_bytes) on `stream`.  Asynchronous.
+int p3dfft_poisson_solve(int nx, int ny, int nz, p3amd::real_t* field, void* work, size_t work_bytes,
+                         void* stream) {
+  try {
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > max_len() || ny > max_len() || nz > max_len()) {
+      set_error("p3dfft_poisson_solve: invalid grid");
+      return -1;
+    }
+    if (work_bytes < poisson_work_bytes(nx, ny, nz)) {
+      set_error("p3dfft_poisson_solve: work area too small");
+      return -1;
+    }
+    State S;
+    poisson_geometry(S, nx, ny, nz);
+    const size_t nc = (size_t)S.nxhpc * ny * nz;
+    cplx_t* spec = (cplx_t*)work;
+    S.w1 = spec + nc;
+    S.w2 = nullptr;  // unpruned, (X,Y,Z) layout, overwrite: never touched
+    hipStream_t st = (hipStream_t)stream;
+    static const unsigned char fwd[4] = {'f', 'f', 't', 0}, bwd[4] = {'t', 'f', 'f', 0};
+    if (!forward_dev(S, field, spec, fwd, st)) return -1;
+    const long long n = (long long)nc;
+    k_poisson<real_t><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(spec, S.nxhpc, ny, nz,
+                                                                    1.0 / ((double)nx * ny * nz));
+    check_hip(hipGetLastError(), "k_poisson");
+    if (!backward_dev(S, spec, field, bwd, st)) return -1;
+    S.w1 = nullptr;
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return -1;
+  }
+}
+
+// Fill n values with i.i.d. N(0,1) from a counter-based generator keyed by seed.
+int p3dfft_amd_fill_normal(p3amd::real_t* p, long long n, unsigned long long seed, void* stream) {
+  if (n <= 0) return 0;
+  k_fill_normal<real_t><<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(p, n, seed);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Streams for callers that run work concurrently (default priority).
+void* p3dfft_amd_stream_create(void) {
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  return (void*)s;
+}
+int p3dfft_amd_stream_destroy(void* s) { return hipStreamDestroy((hipStream_t)s) == hipSuccess ? 0 : -1; }
+int p3dfft_amd_stream_synchronize(void* s) {
+  return hipStreamSynchronize((hipStream_t)s) == hipSuccess ? 0 : -1;
+}
+// 1 when every operation before the event has completed, 0 otherwise.
+int p3dfft_amd_event_query(void* e) { return hipEventQuery((hipEvent_t)e) == hipSuccess ? 1 : 0; }
+
+}  // extern "C"

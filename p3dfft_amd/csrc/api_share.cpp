@@ -176,12 +176,12 @@ int p3dfft_share_ftran_x(const p3amd::real_t* in, p3amd::real_t* send1, void* st
   if (!require_share()) return -1;
   GUARD_BEGIN
   hipStream_t st = (hipStream_t)stream;
-  tstart(5, st);
+  tstart(g, 5, st);
   launch_x_r2c(in, g.w1, (long long)g.jisize * g.kjsize, g.nx, g.nxhp, g.nx, g.nxhpc, st);
-  tstop(5, st);
-  tstart(6, st);
+  tstop(g, 5, st);
+  tstart(g, 6, st);
   share_pack_fcomm1(g.sg, g.w1, (cplx_t*)send1, st);
-  tstop(6, st);
+  tstop(g, 6, st);
   return 0;
   GUARD_END
 }
@@ -192,9 +192,9 @@ int p3dfft_share_ftran_y(const p3amd::real_t* recv1, p3amd::real_t* send2, void*
   if (!require_share()) return -1;
   GUARD_BEGIN
   hipStream_t st = (hipStream_t)stream;
-  tstart(6, st);
+  tstart(g, 6, st);
   share_unpack_fcomm1(g.sg, (const cplx_t*)recv1, g.w2, st);
-  tstop(6, st);
+  tstop(g, 6, st);
   ColSpec y{};
   y.in = g.w2; y.out = g.w2;
   y.si_in = 1; y.sl_in = g.iisize; y.so_in = (long long)g.iisize * g.ny;
@@ -202,12 +202,12 @@ int p3dfft_share_ftran_y(const p3amd::real_t* recv1, p3amd::real_t* send2, void*
   y.n_inner = g.iisize; y.n_outer = g.kjsize; y.n = g.ny; y.kind = COL_C2C; y.fwd = true;
   y.nld = g.ny; y.ld_lo = g.ny; y.ld_hi = g.ny; y.ld_shift = 0;
   y.nst = g.ny; y.st_lo = g.ny; y.st_shift = 0;
-  tstart(7, st);
+  tstart(g, 7, st);
   launch_col(y, st);
-  tstop(7, st);
-  tstart(8, st);
+  tstop(g, 7, st);
+  tstart(g, 8, st);
   share_pack_fcomm2(g.sg, g.w2, (cplx_t*)send2, st);
-  tstop(8, st);
+  tstop(g, 8, st);
   return 0;
   GUARD_END
 }
@@ -232,7 +232,7 @@ int p3dfft_share_ftran_z(p3amd::real_t* recv2, p3amd::real_t* out, const unsigne
   if (kind < 0) { fatal(std::string("Unknown transform type: ") + (char)op[2]); return -1; }
   hipStream_t st = (hipStream_t)stream;
   const long long nxy = (long long)g.iisize * g.jjsize;
-  tstart(8, st);
+  tstart(g, 8, st);
   if (!(none && g.nz == g.nzc && recv2 == out)) {
     ColSpec z{};
     z.in = (const cplx_t*)recv2; z.out = (cplx_t*)out;
@@ -243,7 +243,7 @@ int p3dfft_share_ftran_z(p3amd::real_t* recv2, p3amd::real_t* out, const unsigne
     z.nst = g.nzc; z.st_lo = g.nzcph; z.st_shift = g.nz - g.nzc;
     launch_col(z, st);
   }
-  tstop(8, st);
+  tstop(g, 8, st);
   return 0;
   GUARD_END
 }
@@ -261,7 +261,7 @@ int p3dfft_share_btran_z(p3amd::real_t* in, p3amd::real_t* send, const unsigned 
   hipStream_t st = (hipStream_t)stream;
   const long long nxy = (long long)g.iisize * g.jjsize;
   const int dnz = g.nz - g.nzc;
-  tstart(9, st);
+  tstart(g, 9, st);
   if (!(none && dnz == 0 && in == send)) {
     ColSpec z{};
     z.in = (const cplx_t*)in; z.out = (cplx_t*)send;
@@ -272,7 +272,7 @@ int p3dfft_share_btran_z(p3amd::real_t* in, p3amd::real_t* send, const unsigned 
     z.nst = g.nz; z.st_lo = g.nz; z.st_shift = 0;
     launch_col(z, st);
   }
-  tstop(9, st);
+  tstop(g, 9, st);
   return 0;
   GUARD_END
 }
@@ -283,9 +283,9 @@ int p3dfft_share_btran_y(const p3amd::real_t* recv, p3amd::real_t* send, void* s
   if (!require_share()) return -1;
   GUARD_BEGIN
   hipStream_t st = (hipStream_t)stream;
-  tstart(9, st);
+  tstart(g, 9, st);
   share_unpack_bcomm1(g.sg, (const cplx_t*)recv, g.w2, st);
-  tstop(9, st);
+  tstop(g, 9, st);
   ColSpec y{};
   y.in = g.w2; y.out = g.w2;
   y.si_in = 1; y.sl_in = g.iisize; y.so_in = (long long)g.iisize * g.ny;
@@ -293,12 +293,12 @@ int p3dfft_share_btran_y(const p3amd::real_t* recv, p3amd::real_t* send, void* s
   y.n_inner = g.iisize; y.n_outer = g.kjsize; y.n = g.ny; y.kind = COL_C2C; y.fwd = false;
   y.nld = g.ny; y.ld_lo = g.ny; y.ld_hi = g.ny; y.ld_shift = 0;
   y.nst = g.ny; y.st_lo = g.ny; y.st_shift = 0;
-  tstart(10, st);
+  tstart(g, 10, st);
   launch_col(y, st);
-  tstop(10, st);
-  tstart(11, st);
+  tstop(g, 10, st);
+  tstart(g, 11, st);
   share_pack_bcomm2(g.sg, g.w2, (cplx_t*)send, st);
-  tstop(11, st);
+  tstop(g, 11, st);
   return 0;
   GUARD_END
 }
@@ -309,12 +309,12 @@ int p3dfft_share_btran_x(const p3amd::real_t* recv, p3amd::real_t* out, void* st
   if (!require_share()) return -1;
   GUARD_BEGIN
   hipStream_t st = (hipStream_t)stream;
-  tstart(11, st);
+  tstart(g, 11, st);
   share_unpack_bcomm2(g.sg, (const cplx_t*)recv, g.w1, st);
-  tstop(11, st);
-  tstart(12, st);
+  tstop(g, 11, st);
+  tstart(g, 12, st);
   launch_x_c2r(g.w1, out, (long long)g.jisize * g.kjsize, g.nxhp, g.nx, g.nx, g.nxhp, st);
-  tstop(12, st);
+  tstop(g, 12, st);
   return 0;
   GUARD_END
 }

@@ -30,6 +30,7 @@ struct State {
   // HBM work buffers: w1 (nxhpc, ny, nz), w2 (nxhpc, nyc, nz)
   cplx_t* w1 = nullptr;
   cplx_t* w2 = nullptr;
+  int nv_alloc = 0;  // variables the work buffers hold (grown by the _many calls)
   // share mode (one rank of a P1 x P2 grid): Z pencil work buffer and splits
   bool share = false;
   cplx_t* w3 = nullptr;
@@ -56,19 +57,19 @@ bool fatal(const std::string& msg);
 void notice(const std::string& msg);
 void set_error(const std::string& msg);
 bool require_setup();
-void tstart(int slot, hipStream_t s);
-void tstop(int slot, hipStream_t s);
+void tstart(State& S, int slot, hipStream_t s);
+void tstop(State& S, int slot, hipStream_t s);
 void timers_resolve();
 void map_data_to_proc(int data, int proc, int* st, int* en, int* sz);
 bool compute_geometry(State& s, const int dims[2], int ipid, int jpid);
 size_t real_elems();
 size_t spec_elems();
 void ensure_staging();
-bool forward_dev(const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s);
-bool backward_dev(cplx_t* in, real_t* out, const unsigned char* op, hipStream_t s);
-bool cheby_dev(const real_t* in, cplx_t* out, double Lz, hipStream_t s);
+bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s);
+bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hipStream_t s);
+bool cheby_dev(State& S, const real_t* in, cplx_t* out, double Lz, hipStream_t s);
 void release_state();
-int chunk_planes();
+int chunk_planes(State& S);
 void share_release();
 
 }  // namespace p3amd

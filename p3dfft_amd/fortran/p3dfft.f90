@@ -22,7 +22,8 @@ module p3dfft
   real(r8), save, public :: timers(12) = 0.0_r8
 
   public :: p3dfft_setup, p3dfft_get_dims, p3dfft_ftran_r2c, p3dfft_btran_c2r, &
-            p3dfft_cheby, p3dfft_clean, get_timers, set_timers
+            p3dfft_cheby, p3dfft_ftran_r2c_many, p3dfft_btran_c2r_many, p3dfft_cheby_many, &
+            p3dfft_clean, get_timers, set_timers
 
   interface
     subroutine c_p3dfft_setup(dims, nx, ny, nz, comm, nxc, nyc, nzc, ow, memsize) &
@@ -47,6 +48,23 @@ module p3dfft
     subroutine c_p3dfft_cheby(a, b, lz) bind(C, name='p3dfft_cheby')
       import :: c_ptr
       type(c_ptr), value :: a, b, lz
+    end subroutine
+    subroutine c_p3dfft_ftran_r2c_many(a, dim_in, b, dim_out, nv, op) bind(C, name='p3dfft_ftran_r2c_many')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: a, b
+      integer(c_int) :: dim_in, dim_out, nv
+      character(kind=c_char) :: op(*)
+    end subroutine
+    subroutine c_p3dfft_btran_c2r_many(a, dim_in, b, dim_out, nv, op) bind(C, name='p3dfft_btran_c2r_many')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: a, b
+      integer(c_int) :: dim_in, dim_out, nv
+      character(kind=c_char) :: op(*)
+    end subroutine
+    subroutine c_p3dfft_cheby_many(a, dim_in, b, dim_out, nv, lz) bind(C, name='p3dfft_cheby_many')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: a, b, lz
+      integer(c_int) :: dim_in, dim_out, nv
     end subroutine
     subroutine c_p3dfft_clean() bind(C, name='p3dfft_clean')
     end subroutine
@@ -117,6 +135,44 @@ contains
     real(p3dfft_type), target :: lzc
     lzc = Lz
     call c_p3dfft_cheby(c_loc(in(1)), c_loc(out(1)), c_loc(lzc))
+    call c_get_timers(timers)
+  end subroutine
+
+  ! build/ftran.F90:104 (variable j of XgYZ at j*dim_in reals, of XYZg at
+  ! j*dim_out complex elements)
+  subroutine p3dfft_ftran_r2c_many(XgYZ, dim_in, XYZg, dim_out, nv, op)
+    integer, intent(in) :: dim_in, dim_out, nv
+    real(p3dfft_type), target :: XgYZ(*)
+    complex(p3dfft_type), target :: XYZg(*)
+    character(len=*), intent(in) :: op
+    integer(c_int) :: di, do_, n
+    di = dim_in; do_ = dim_out; n = nv
+    call c_p3dfft_ftran_r2c_many(c_loc(XgYZ(1)), di, c_loc(XYZg(1)), do_, n, op(1:3)//c_null_char)
+    call c_get_timers(timers)
+  end subroutine
+
+  ! build/btran.F90:104
+  subroutine p3dfft_btran_c2r_many(XYZg, dim_in, XgYZ, dim_out, nv, op)
+    integer, intent(in) :: dim_in, dim_out, nv
+    complex(p3dfft_type), target :: XYZg(*)
+    real(p3dfft_type), target :: XgYZ(*)
+    character(len=*), intent(in) :: op
+    integer(c_int) :: di, do_, n
+    di = dim_in; do_ = dim_out; n = nv
+    call c_p3dfft_btran_c2r_many(c_loc(XYZg(1)), di, c_loc(XgYZ(1)), do_, n, op(1:3)//c_null_char)
+    call c_get_timers(timers)
+  end subroutine
+
+  ! build/ftran.F90:339
+  subroutine p3dfft_cheby_many(in, dim_in, out, dim_out, nv, Lz)
+    integer, intent(in) :: dim_in, dim_out, nv
+    real(p3dfft_type), target :: in(*)
+    complex(p3dfft_type), target :: out(*)
+    real(p3dfft_type), intent(in) :: Lz
+    real(p3dfft_type), target :: lzc
+    integer(c_int) :: di, do_, n
+    di = dim_in; do_ = dim_out; n = nv; lzc = Lz
+    call c_p3dfft_cheby_many(c_loc(in(1)), di, c_loc(out(1)), do_, n, c_loc(lzc))
     call c_get_timers(timers)
   end subroutine
 

@@ -1,0 +1,229 @@
+"""Continuous-batching service of spectral Poisson solves on one MI355X.
+
+Requests (mixed grid shapes) arrive over time with their right-hand sides
+resident in HBM.  A request is admitted as soon as it has arrived, a stream
+slot is free and its work area fits the HBM budget (first-fit arena); its
+solve (r2c, -1/|k|^2 multiply, c2r: p3dfft_poisson_solve) is queued on the
+slot's stream, so admitted solves of different shapes overlap on the device.
+Completion is detected with HIP events, whose GPU timestamps give each
+request's latency (arrival to completion).  The same loop with one slot and a
+synchronise after every launch is the one-at-a-time baseline.
+"""
+import ctypes
+import time
+from collections import deque
+
+import numpy as np
+
+from . import P3DFFTError, load
+
+GRID_SIZES = (32, 48, 64, 96, 128, 160, 192, 256, 320, 384)
+
+
+def _bind(L):
+    if getattr(L, "_serve_bound", False):
+        return L
+    vp = ctypes.c_void_p
+    L.p3dfft_poisson_work_bytes.argtypes = [ctypes.c_int] * 3
+    L.p3dfft_poisson_work_bytes.restype = ctypes.c_size_t
+    L.p3dfft_poisson_solve.argtypes = [ctypes.c_int] * 3 + [vp, vp, ctypes.c_size_t, vp]
+    L.p3dfft_poisson_solve.restype = ctypes.c_int
+    L.p3dfft_amd_fill_normal.argtypes = [vp, ctypes.c_longlong, ctypes.c_ulonglong, vp]
+    L.p3dfft_amd_fill_normal.restype = ctypes.c_int
+    L.p3dfft_amd_stream_create.restype = vp
+    L.p3dfft_amd_stream_destroy.argtypes = [vp]
+    L.p3dfft_amd_stream_synchronize.argtypes = [vp]
+    L.p3dfft_amd_event_query.argtypes = [vp]
+    L.p3dfft_amd_event_query.restype = ctypes.c_int
+    L.p3dfft_amd_prepare_length.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_prepare_length.restype = ctypes.c_int
+    L._serve_bound = True
+    return L
+
+
+class Arena:
+    """First-fit allocator over one HBM allocation (offsets in bytes, 256-aligned)."""
+
+    def __init__(self, lib, nbytes):
+        self.lib = lib
+        self.size = int(nbytes)
+        self.base = lib.p3dfft_amd_malloc(self.size) if self.size else None
+        if self.size and not self.base:
+            raise P3DFFTError("arena allocation of %d bytes failed" % self.size)
+        self.free_list = [(0, self.size)]
+        self.used = 0
+        self.peak = 0
+
+    def alloc(self, n):
+        n = (int(n) + 255) & ~255
+        for i, (off, sz) in enumerate(self.free_list):
+            if sz >= n:
+                if sz == n:
+                    self.free_list.pop(i)
+                else:
+                    self.free_list[i] = (off + n, sz - n)
+                self.used += n
+                self.peak = max(self.peak, self.used)
+                return (off, n)
+        return None
+
+    def release(self, blk):
+        off, n = blk
+        self.used -= n
+        fl = self.free_list
+        fl.append((off, n))
+        fl.sort()
+        merged = [fl[0]]
+        for o, s in fl[1:]:
+            po, ps = merged[-1]
+            if po + ps == o:
+                merged[-1] = (po, ps + s)
+            else:
+                merged.append((o, s))
+        self.free_list = merged
+
+    def ptr(self, blk):
+        return self.base + blk[0]
+
+    def close(self):
+        if self.base:
+            self.lib.p3dfft_amd_free(self.base)
+            self.base = None
+
+
+def make_requests(n, seed=2000, sizes=GRID_SIZES):
+    """Grid shapes drawn uniformly per axis from `sizes` (mixed x, y, z)."""
+    rng = np.random.default_rng(seed)
+    return [tuple(int(v) for v in rng.choice(sizes, 3)) for _ in range(n)]
+
+
+def exp_arrivals(n, mean_gap, seed=2001):
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(mean_gap, size=n)) - 0.0
+
+
+class PoissonServer:
+    """Fields of all requests resident in one HBM allocation; solves run on
+    `slots` streams with work areas from a first-fit arena of `work_budget`
+    bytes (the HBM left for in-flight solves)."""
+
+    def __init__(self, shapes, precision=4, slots=4, work_budget=None, seed=7):
+        self.lib = _bind(load(precision))
+        self.precision = precision
+        self.rsize = 4 if precision == 4 else 8
+        self.shapes = list(shapes)
+        self.nelem = [a * b * c for (a, b, c) in self.shapes]
+        self.field_off = np.concatenate([[0], np.cumsum([((e * self.rsize + 255) & ~255) for e in self.nelem])])
+        self.fields = self.lib.p3dfft_amd_malloc(int(self.field_off[-1]))
+        if not self.fields:
+            raise P3DFFTError("field allocation failed")
+        self.work = [int(self.lib.p3dfft_poisson_work_bytes(*s)) for s in self.shapes]
+        if work_budget is None:
+            work_budget = slots * max(self.work)
+        if work_budget < max(self.work):
+            raise P3DFFTError("work budget below the largest request's work area")
+        self.arena = Arena(self.lib, work_budget)
+        self.slots = slots
+        self.streams = [self.lib.p3dfft_amd_stream_create() for _ in range(slots)]
+        self.events = [self.lib.p3dfft_amd_event_create() for _ in range(slots)]
+        self.t0ev = self.lib.p3dfft_amd_event_create()
+        self.seed = seed
+        # plans and twiddles of every length are built before serving
+        for n in sorted({v for sh in self.shapes for v in sh}):
+            if self.lib.p3dfft_amd_prepare_length(n):
+                raise P3DFFTError("cannot prepare length %d" % n)
+
+    def field_ptr(self, i):
+        return self.fields + int(self.field_off[i])
+
+    def fill(self, idx=None):
+        """Right-hand sides: i.i.d. N(0,1) per request (counter-based, seeded by
+        request id).  The solve zeroes the k=0 mode, which removes the mean."""
+        for i in (range(len(self.shapes)) if idx is None else idx):
+            self.lib.p3dfft_amd_fill_normal(self.field_ptr(i), self.nelem[i], self.seed * 1000003 + i, None)
+        self.lib.p3dfft_amd_synchronize()
+
+    def download(self, i):
+        nx, ny, nz = self.shapes[i]
+        dt = np.float32 if self.precision == 4 else np.float64
+        out = np.empty((nz, ny, nx), dtype=dt)
+        self.lib.p3dfft_amd_memcpy_d2h(out.ctypes.data, self.field_ptr(i), out.nbytes)
+        return out
+
+    def solve_alone(self, i):
+        blk = self.arena.alloc(self.work[i])
+        try:
+            rc = self.lib.p3dfft_poisson_solve(*self.shapes[i], self.field_ptr(i), self.arena.ptr(blk),
+                                               self.work[i], self.streams[0])
+            if rc:
+                raise P3DFFTError(self.lib.p3dfft_amd_last_error().decode())
+            self.lib.p3dfft_amd_stream_synchronize(self.streams[0])
+        finally:
+            self.arena.release(blk)
+
+    def run(self, order, arrivals, sequential=False):
+        """Serve requests `order` (indices) arriving at `arrivals` seconds.
+        Returns per-request latency (s) and the makespan (s)."""
+        L = self.lib
+        n = len(order)
+        nslots = 1 if sequential else self.slots
+        lat = np.zeros(n)
+        pending = deque(range(n))
+        busy = {}  # slot -> (k, blk)
+        free_slots = list(range(nslots))
+        done = 0
+        max_used = 0
+        L.p3dfft_amd_synchronize()
+        L.p3dfft_amd_event_record(self.t0ev, None)
+        L.p3dfft_amd_synchronize()
+        t_start = time.perf_counter()
+        last_done_ms = 0.0
+        while done < n:
+            progress = False
+            for slot in list(busy):
+                k, blk = busy[slot]
+                if L.p3dfft_amd_event_query(self.events[slot]):
+                    t_ms = L.p3dfft_amd_event_elapsed_ms(self.t0ev, self.events[slot])
+                    lat[k] = max(t_ms * 1e-3 - arrivals[k], 0.0)
+                    last_done_ms = max(last_done_ms, t_ms)
+                    self.arena.release(blk)
+                    del busy[slot]
+                    free_slots.append(slot)
+                    done += 1
+                    progress = True
+            now = time.perf_counter() - t_start
+            while pending and free_slots and arrivals[pending[0]] <= now:
+                k = pending[0]
+                i = order[k]
+                blk = self.arena.alloc(self.work[i])
+                if blk is None:
+                    break  # head of the queue waits for memory (FIFO admission)
+                pending.popleft()
+                slot = free_slots.pop()
+                st = self.streams[slot]
+                rc = L.p3dfft_poisson_solve(*self.shapes[i], self.field_ptr(i), self.arena.ptr(blk), self.work[i], st)
+                if rc:
+                    raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+                L.p3dfft_amd_event_record(self.events[slot], st)
+                busy[slot] = (k, blk)
+                max_used = max(max_used, self.arena.used)
+                progress = True
+                if sequential:
+                    L.p3dfft_amd_stream_synchronize(st)
+            if not progress:
+                time.sleep(2e-5)
+        makespan = last_done_ms * 1e-3 - float(arrivals[0])
+        return lat, makespan, max_used
+
+    def close(self):
+        L = self.lib
+        L.p3dfft_amd_synchronize()
+        for s in self.streams:
+            L.p3dfft_amd_stream_destroy(s)
+        for e in self.events + [self.t0ev]:
+            L.p3dfft_amd_event_destroy(e)
+        self.arena.close()
+        if self.fields:
+            L.p3dfft_amd_free(self.fields)
+            self.fields = None
+

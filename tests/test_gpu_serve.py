@@ -1,0 +1,95 @@
+"""Served spectral Poisson solves: each result equals the same request solved
+alone, the solution matches an FP64 numpy solve (the FP32 path's stated
+tolerance 2e-6 relative L2 times a margin of 10 for the operator), admission
+never exceeds the HBM work budget, and the sequential baseline gives the same
+results."""
+import numpy as np
+import pytest
+
+import p3dfft_amd
+from p3dfft_amd import serve
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    assert p3dfft_amd.device_count() > 0, "no GPU visible to the HIP runtime"
+
+
+def fp64_solve(f):
+    """FP64 numpy solve: u_hat = -f_hat/|k|^2, k=0 -> 0."""
+    nz, ny, nx = f.shape
+    F = np.fft.rfftn(f.astype(np.float64))
+    kx = np.arange(nx // 2 + 1)
+    ky = np.fft.fftfreq(ny, 1.0 / ny)
+    kz = np.fft.fftfreq(nz, 1.0 / nz)
+    k2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    mult = np.where(k2 > 0, -1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
+    return np.fft.irfftn(F * mult, s=(nz, ny, nx), axes=(0, 1, 2))
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_served_equals_alone_and_fp64(precision):
+    sizes = (32, 48, 64, 96, 128)
+    shapes = serve.make_requests(40, seed=11, sizes=sizes)
+    S = serve.PoissonServer(shapes, precision=precision, slots=4)
+    try:
+        S.fill()
+        rhs = [S.download(i) for i in range(len(shapes))]
+        arr = serve.exp_arrivals(len(shapes), 50e-6, seed=12)
+        lat, makespan, _ = S.run(list(range(len(shapes))), arr)
+        assert np.all(lat >= 0) and makespan > 0
+        mixed = [S.download(i) for i in range(len(shapes))]
+        S.fill()
+        for i in range(len(shapes)):
+            S.solve_alone(i)
+            alone = S.download(i)
+            assert np.array_equal(alone, mixed[i]), i
+            tol = 2e-6 if precision == 4 else 1e-12
+            assert rel(mixed[i], fp64_solve(rhs[i].astype(np.float64))) < tol * 10, (i, shapes[i])
+    finally:
+        S.close()
+
+
+def test_admission_respects_work_budget():
+    """A budget that holds the largest request once: requests queue for memory,
+    the arena never exceeds it, and results equal the unconstrained run."""
+    shapes = serve.make_requests(24, seed=21, sizes=(32, 64, 96, 128, 160))
+    S0 = serve.PoissonServer(shapes, precision=4, slots=4)
+    try:
+        S0.fill()
+        S0.run(list(range(len(shapes))), np.zeros(len(shapes)))
+        ref = [S0.download(i) for i in range(len(shapes))]
+        work = list(S0.work)
+    finally:
+        S0.close()
+    budget = max(work) + 4096
+    S = serve.PoissonServer(shapes, precision=4, slots=4, work_budget=budget)
+    try:
+        S.fill()
+        _, _, max_used = S.run(list(range(len(shapes))), np.zeros(len(shapes)))
+        assert max_used <= budget and S.arena.peak <= budget
+        for i in range(len(shapes)):
+            assert np.array_equal(S.download(i), ref[i])
+    finally:
+        S.close()
+
+
+def test_sequential_baseline_same_results():
+    shapes = serve.make_requests(12, seed=31, sizes=(32, 48, 64))
+    S = serve.PoissonServer(shapes, precision=4, slots=4)
+    try:
+        S.fill()
+        S.run(list(range(len(shapes))), np.zeros(len(shapes)))
+        a = [S.download(i) for i in range(len(shapes))]
+        S.fill()
+        S.run(list(range(len(shapes))), np.zeros(len(shapes)), sequential=True)
+        for i in range(len(shapes)):
+            assert np.array_equal(S.download(i), a[i])
+    finally:
+        S.close()
