@@ -141,6 +141,9 @@ int p3dfft_share_btran_x(const p3dfft_real *recv, p3dfft_real *out, void *stream
  * p3dfft_setup; any number of solves may be in flight on different streams. */
 size_t p3dfft_poisson_work_bytes(int nx, int ny, int nz);
 int p3dfft_poisson_solve(int nx, int ny, int nz, p3dfft_real *field, void *work, size_t work_bytes, void *stream);
+/* 1 (default): Z forward, multiply and Z inverse fused in one kernel; 0: the
+ * composition of the forward transform, a multiply pass and the backward one */
+void p3dfft_poisson_set_fused(int on);
 /* build the plan and twiddle table of a transform length ahead of time */
 int p3dfft_amd_prepare_length(int n);
 /* i.i.d. N(0,1) values from a counter-based generator keyed by seed */

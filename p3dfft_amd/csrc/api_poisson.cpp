@@ -67,6 +67,10 @@ static void poisson_geometry(State& S, int nx, int ny, int nz) {
   compute_geometry(S, dims, 0, 0);
 }
 
+// fused Z forward / multiply / Z inverse kernel (default on); off composes
+// the solve from the forward transform, a multiply pass and the backward one
+static bool g_fused_solve = true;
+
 size_t poisson_work_bytes(int nx, int ny, int nz) {
   // spectrum + X/Y intermediate
   return 2 * (size_t)(nx / 2 + 1) * ny * nz * sizeof(cplx_t);
@@ -88,6 +92,10 @@ int p3dfft_amd_prepare_length(int n) {
     return -1;
   }
 }
+
+// 1 (default): fused Z-solve kernel; 0: forward transform, multiply pass,
+// backward transform (the composition the fused kernel replaces).
+void p3dfft_poisson_set_fused(int on) { g_fused_solve = (on != 0); }
 
 // HBM work bytes one solve of this shape needs besides its field.
 size_t p3dfft_poisson_work_bytes(int nx, int ny, int nz) { return poisson_work_bytes(nx, ny, nz); }
@@ -112,12 +120,43 @@ int p3dfft_poisson_solve(int nx, int ny, int nz, p3amd::real_t* field, void* wor
     S.w1 = spec + nc;
     S.w2 = nullptr;  // unpruned, (X,Y,Z) layout, overwrite: never touched
     hipStream_t st = (hipStream_t)stream;
-    static const unsigned char fwd[4] = {'f', 'f', 't', 0}, bwd[4] = {'t', 'f', 'f', 0};
-    if (!forward_dev(S, field, spec, fwd, st)) return -1;
+    const double scale = 1.0 / ((double)nx * ny * nz);
+    if (g_fused_solve) {
+      // X r2c, Y c2c, fused Z forward / multiply / Z inverse, Y c2c, X c2r
+      const long long nxy = (long long)S.nxhpc * ny;
+      launch_x_r2c(field, S.w1, (long long)ny * nz, nx, S.nxhpc, nx, S.nxhpc, st);
+      ColSpec y{};
+      y.in = S.w1; y.out = spec;
+      y.si_in = 1; y.sl_in = S.nxhpc; y.so_in = nxy;
+      y.si_out = 1; y.sl_out = S.nxhpc; y.so_out = nxy;
+      y.n_inner = S.nxhpc; y.n_outer = nz; y.n = ny; y.kind = COL_C2C; y.fwd = true;
+      y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
+      y.nst = ny; y.st_lo = ny; y.st_shift = 0;
+      launch_col(y, st);
+      if (launch_zsolve(spec, S.nxhpc, ny, nz, scale, st)) {
+        y.in = spec; y.out = S.w1; y.fwd = false;
+        launch_col(y, st);
+        launch_x_c2r(S.w1, field, (long long)ny * nz, S.nxhpc, nx, nx, S.nxhpc, st);
+        S.w1 = nullptr;
+        return 0;
+      }
+      // no specialised Z length: Z forward here, then the composed tail below
+      ColSpec z{};
+      z.in = spec; z.out = spec;
+      z.n_inner = (int)nxy; z.n_outer = 1;
+      z.si_in = 1; z.sl_in = nxy; z.si_out = 1; z.sl_out = nxy;
+      z.n = nz; z.kind = COL_C2C; z.fwd = true;
+      z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
+      z.nst = nz; z.st_lo = nz; z.st_shift = 0;
+      launch_col(z, st);
+    } else {
+      static const unsigned char fwd[4] = {'f', 'f', 't', 0};
+      if (!forward_dev(S, field, spec, fwd, st)) return -1;
+    }
     const long long n = (long long)nc;
-    k_poisson<real_t><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(spec, S.nxhpc, ny, nz,
-                                                                    1.0 / ((double)nx * ny * nz));
+    k_poisson<real_t><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(spec, S.nxhpc, ny, nz, scale);
     check_hip(hipGetLastError(), "k_poisson");
+    static const unsigned char bwd[4] = {'t', 'f', 'f', 0};
     if (!backward_dev(S, spec, field, bwd, st)) return -1;
     S.w1 = nullptr;
     return 0;

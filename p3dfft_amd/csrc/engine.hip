@@ -500,6 +500,29 @@ __global__ __launch_bounds__(NT) void k_col_ct(ColArgs a) {
   }
 }
 
+// fused Z-solve: the Z column of a spectrum (nxh, ny, nz), forward FFT,
+// -scale/|k|^2, inverse FFT, in place
+template <class T, int TX, int NT, class RL>
+__global__ __launch_bounds__(NT) void k_zsolve_ct(ColArgs a, double scale, int nxh, int ny) {
+  typedef typename TwT<T>::type TwL;
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const long long i0 = (long long)t * TX;
+  const int nb = (int)((long long)a.n_inner - i0 < TX ? (long long)a.n_inner - i0 : TX);
+  const TwL* tw = (const TwL*)a.tw;
+  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  ColIO<T> io;
+  io.in = (const cpx<T>*)a.in + i0;
+  io.out = (cpx<T>*)a.out + i0;
+  io.si_in = 1; io.sl_in = a.sl_in; io.si_out = 1; io.sl_out = a.sl_out;
+  io.nb = nb;
+  io.ld_lo = a.L.n; io.ld_hi = a.L.n; io.ld_shift = 0;
+  io.nst = a.L.n; io.st_lo = a.L.n; io.st_shift = 0;
+  SolveCtx sc;
+  sc.i0 = i0; sc.nxh = nxh; sc.ny = ny; sc.nz = a.L.n; sc.scale = scale;
+  run_fft_solve<T, TX, NT>(lds, tw, io, sc, RL{});
+}
+
 // ---------------------------------------------------------------------------
 // small kernels
 // ---------------------------------------------------------------------------
@@ -770,6 +793,7 @@ struct CtInfo {
   void (*col)(bool, const ColArgs&, unsigned, size_t, hipStream_t);
   void (*xr2c)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*xc2r)(const XArgs&, unsigned, size_t, hipStream_t);
+  void (*zsolve)(const ColArgs&, double, int, int, unsigned, size_t, hipStream_t);
 };
 
 template <int... Rs>
@@ -801,6 +825,11 @@ struct CtLen {
     (void)once;
     k_x_c2r_ct<real_t, BX, NTX, RL><<<g, NTX, bytes, s>>>(a);
   }
+  static void zsolve(const ColArgs& a, double scale, int nxh, int ny, unsigned g, size_t bytes, hipStream_t s) {
+    static bool once = (allow_big_lds(k_zsolve_ct<real_t, TX, NTC, RL>), true);
+    (void)once;
+    k_zsolve_ct<real_t, TX, NTC, RL><<<g, NTC, bytes, s>>>(a, scale, nxh, ny);
+  }
   static CtInfo info() {
     CtInfo c;
     std::memset(&c, 0, sizeof(c));
@@ -808,7 +837,7 @@ struct CtLen {
     const int rs[] = {Rs...};
     c.nr = (int)sizeof...(Rs);
     for (int i = 0; i < c.nr; i++) c.radix[i] = rs[i];
-    c.col = &col; c.xr2c = &xr2c; c.xc2r = &xc2r;
+    c.col = &col; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve;
     return c;
   }
 };
@@ -992,6 +1021,32 @@ void launch_col(const ColSpec& c, hipStream_t st) {
   if (fwd) dispatch_nt<GoCol<true>>(nt, a, (unsigned)nwg, bytes, st);
   else dispatch_nt<GoCol<false>>(nt, a, (unsigned)nwg, bytes, st);
   check_hip(hipGetLastError(), "k_col launch");
+}
+
+bool launch_zsolve(cplx_t* spec, int nxh, int ny, int nz, double scale, hipStream_t st) {
+  const Len1D& L = get_len(nz);
+  const CtInfo* ct = g_use_ct ? find_ct(L) : nullptr;
+  if (!ct) return false;
+  ColArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const long long ninner = (long long)nxh * ny;
+  if (ninner > 0x7fffffffLL) return false;
+  a.in = spec; a.out = spec;
+  a.si_in = 1; a.sl_in = ninner; a.si_out = 1; a.sl_out = ninner;
+  a.n_inner = (int)ninner; a.n_outer = 1;
+  a.kind = COL_C2C;
+  const int TX = ct->TX;
+  a.TX = TX;
+  a.tiles = (int)((ninner + TX - 1) / TX);
+  a.L = L.d;
+  a.tw = L.tw;
+  const size_t data = (size_t)TX * nz * sizeof(cplx_t);
+  size_t bytes = lds_layout(data, L, &a.tw_lds, &a.tw_off);
+  if (bytes > 80 * 1024) { a.tw_lds = 0; a.tw_off = 0; bytes = data; }
+  if (bytes > kLdsMax) return false;
+  ct->zsolve(a, scale, nxh, ny, (unsigned)a.tiles, bytes, st);
+  check_hip(hipGetLastError(), "k_zsolve_ct launch");
+  return true;
 }
 
 void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long ks, double scale, double L,

@@ -71,6 +71,11 @@ struct ColSpec {
 };
 void launch_col(const ColSpec& c, hipStream_t st);
 
+// Fused Z-direction spectral solve on a spectrum (nxh, ny, nz) in place:
+// forward FFT along z, multiply by -scale/|k|^2 (k=0 -> 0), inverse FFT.
+// Returns false when nz has no specialised kernel (the caller composes it).
+bool launch_zsolve(cplx_t* spec, int nxh, int ny, int nz, double scale, hipStream_t st);
+
 // Chebyshev post-processing of a 'ffc' spectrum (build/ftran.F90:408-451):
 // scale, then the derivative recurrence along z.  Element (col c, z k) at
 // out[c*cs + k*ks].

@@ -178,6 +178,38 @@ __device__ __forceinline__ void run_fft_io(cpx<T>* lds, const typename TwT<T>::t
   mid_and_last<T, FWD, N, B, NT, R0, 0, Rs...>(lds, tw, io);
 }
 
+// Spectral solve along the column: forward FFT, per-mode multiply by
+// -scale/|k|^2 (k=0 -> 0), inverse FFT, all in one LDS tile.  The inner index
+// of column b is i0 + b = kx + nxh * ky_index; the row is the kz index.
+struct SolveCtx {
+  long long i0;
+  int nxh, ny, nz;
+  double scale;
+};
+
+template <class T, int B, int NT, int R0, int... Rs>
+__device__ __forceinline__ void run_fft_solve(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO<T>& io,
+                                              const SolveCtx& sc, RList<R0, Rs...>) {
+  static_assert(sizeof...(Rs) >= 1, "at least two stages");
+  constexpr int N = R0 * RProd<Rs...>::value;
+  stage0_global<T, R0, true, N, B, NT>(lds, io);
+  stages_ct<T, true, N, B, B, NT, R0, 0, Rs...>(lds, tw);
+  for (int q = threadIdx.x; q < B * N; q += NT) {
+    const int e = q / B, b = q % B;
+    const long long i = sc.i0 + b;
+    const int kx = (int)(i % sc.nxh);
+    const int y = (int)(i / sc.nxh);
+    const double ky = y <= sc.ny / 2 ? y : y - sc.ny;
+    const double kz = e <= sc.nz / 2 ? e : e - sc.nz;
+    const double k2 = (double)kx * kx + ky * ky + kz * kz;
+    const double f = k2 > 0.0 ? -sc.scale / k2 : 0.0;
+    const cpx<T> v = lds[q];
+    lds[q] = mk<T>((T)(v.x * f), (T)(v.y * f));
+  }
+  __syncthreads();
+  mid_and_last<T, false, N, B, NT, 1, 0, R0, Rs...>(lds, tw, io);
+}
+
 // Twiddle table entries used by a radix list (all stages after the first).
 template <int... Rs>
 struct TwCount;

@@ -35,6 +35,7 @@ def _bind(L):
     L.p3dfft_amd_stream_synchronize.argtypes = [vp]
     L.p3dfft_amd_event_query.argtypes = [vp]
     L.p3dfft_amd_event_query.restype = ctypes.c_int
+    L.p3dfft_poisson_set_fused.argtypes = [ctypes.c_int]
     L.p3dfft_amd_prepare_length.argtypes = [ctypes.c_int]
     L.p3dfft_amd_prepare_length.restype = ctypes.c_int
     L._serve_bound = True

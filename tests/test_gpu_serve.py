@@ -93,3 +93,37 @@ def test_sequential_baseline_same_results():
             assert np.array_equal(S.download(i), a[i])
     finally:
         S.close()
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_fused_zsolve_matches_composed(precision):
+    """The fused Z forward / multiply / Z inverse kernel against the composition
+    of the forward transform, the multiply pass and the backward transform
+    (the same butterflies and the same multiply, so equal to a few ulps), on
+    Z lengths with a specialised kernel and without one (40, 27), and both
+    against the FP64 numpy solve."""
+    shapes = [(32, 48, 64), (48, 40, 40), (64, 32, 27), (20, 24, 512), (96, 36, 128)]
+    S = serve.PoissonServer(shapes, precision=precision, slots=2)
+    L = S.lib
+    try:
+        S.fill()
+        rhs = [S.download(i) for i in range(len(shapes))]
+        fused = []
+        for i in range(len(shapes)):
+            S.solve_alone(i)
+            fused.append(S.download(i))
+        S.fill()
+        L.p3dfft_poisson_set_fused(0)
+        try:
+            for i in range(len(shapes)):
+                S.solve_alone(i)
+                comp = S.download(i)
+                scale = np.abs(comp).max()
+                ulp = np.finfo(np.float32 if precision == 4 else np.float64).eps
+                assert np.abs(fused[i] - comp).max() <= 64 * ulp * scale, shapes[i]
+                tol = 2e-6 if precision == 4 else 1e-12
+                assert rel(fused[i], fp64_solve(rhs[i].astype(np.float64))) < tol * 10, shapes[i]
+        finally:
+            L.p3dfft_poisson_set_fused(1)
+    finally:
+        S.close()
