@@ -157,7 +157,7 @@ def serve_leg(n_req, slots, load_frac=0.7):
     """Mixed-size spectral Poisson solves (grid sizes drawn per axis from
     {32,...,384}), FP32 with compensated twiddles, fields resident in HBM.
     Throughput at saturation (all requests queued at t=0) for the served loop
-    and for one-at-a-time solves; p50/p99 latency of the served loop with
+    and for one-at-a-time solves (FP32, and the plain FP64 path); p50/p99 latency of the served loop with
     exponential inter-arrival times at load_frac of its capacity."""
     from p3dfft_amd import serve
     shapes = serve.make_requests(n_req)
@@ -186,8 +186,20 @@ def serve_leg(n_req, slots, load_frac=0.7):
         lat_q, _, _ = S.run(order, arr, sequential=True)
     finally:
         S.close()
+    # the plain FP64 GPU path, one solve at a time, on the same requests
+    S8 = serve.PoissonServer(shapes, precision=8, slots=1)
+    try:
+        S8.fill()
+        S8.run(order[:min(50, n_req)], np.zeros(min(50, n_req)), sequential=True)  # warm
+        S8.fill()
+        _, mk_seq64, _ = S8.run(order, np.zeros(n_req), sequential=True)
+        thr_seq64 = n_req / mk_seq64
+    finally:
+        S8.close()
     return {"requests": n_req, "slots": slots, "solves_per_s": thr_served, "one_at_a_time_solves_per_s": thr_seq,
             "speedup_vs_one_at_a_time": thr_served / thr_seq,
+            "one_at_a_time_plain_fp64_solves_per_s": thr_seq64,
+            "speedup_vs_one_at_a_time_plain_fp64": thr_served / thr_seq64,
             "latency_load": "exponential inter-arrival at %.0f%% of served capacity (%.0f req/s)"
                             % (100 * load_frac, load_frac * thr_served),
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
@@ -212,7 +224,7 @@ def main():
                     help="record no per-stage HIP events (whole-step time only)")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests of the served Poisson leg (0 = skip)")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--field", choices=["modes", "random"], default="modes",
                     help="modes: the 8-mode + noise field; random: N(0,1) (fast, for profiler runs)")
     args = ap.parse_args()

@@ -338,18 +338,29 @@ __global__ __launch_bounds__(NT) void k_col(ColArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// compile-time specialised kernels (same arguments and semantics as above)
+// compile-time specialised kernels (same arguments and semantics as above).
+// TWL: the twiddle table is staged in LDS (decided per length at compile
+// time, so twiddle reads are LDS reads).  Addresses are a wave-uniform 64-bit
+// tile base plus 32-bit byte offsets (the host checks that they fit).
 // ---------------------------------------------------------------------------
-template <class T, int B, int NT, class RL>
-__global__ __launch_bounds__(NT) void k_x_r2c_ct(XArgs a) {
+template <class T, int NT, bool TWL>
+__device__ __forceinline__ const typename TwT<T>::type* ct_twiddles(const void* tw, int off, int count) {
   typedef typename TwT<T>::type TwL;
+  if constexpr (TWL) return stage_twiddles<T, NT>(g_smem, off, (const TwL*)tw, count);
+  else return (const TwL*)tw;
+}
+
+template <class T, int B, int NT, class RL, int W, bool TWL>
+__global__ __launch_bounds__(NT, W) void k_x_r2c_ct(XArgs a) {
   constexpr int N = RL_N<RL>::value;
   constexpr int BP = B + 1;
   cpx<T>* lds = (cpx<T>*)g_smem;
-  const TwL* tw = (const TwL*)a.tw;
-  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
   const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
-  const T* in = (const T*)a.in;
+  // rows of this tile (2B unless it is the last one)
+  const int nr = (int)(a.nrows - 2 * pair0 < 2 * B ? a.nrows - 2 * pair0 : 2 * B);
+  const char* in = (const char*)a.in + 2 * pair0 * a.si_row * (long long)sizeof(T);
+  const unsigned si = (unsigned)(a.si_row * sizeof(T));
   {
     // issue every load of the tile before the LDS writes
     constexpr int LI = (B * N + NT - 1) / NT;
@@ -358,10 +369,15 @@ __global__ __launch_bounds__(NT) void k_x_r2c_ct(XArgs a) {
     for (int i = 0; i < LI; i++) {
       const int q = threadIdx.x + i * NT;
       const int b = q / N, j = q % N;
-      const long long ra = 2 * (pair0 + b), rb = ra + 1;
       const bool ok = ((B * N) % NT == 0) || q < B * N;
-      xa[i] = (ok && ra < a.nrows) ? in[ra * a.si_row + j] : (T)0;
-      xb[i] = (ok && rb < a.nrows) ? in[rb * a.si_row + j] : (T)0;
+      // out-of-range items load the tile's first element (always valid) and
+      // are zeroed, so no load sits under a branch
+      const bool oka = ok && 2 * b < nr, okb = ok && 2 * b + 1 < nr;
+      const unsigned o = (unsigned)(2 * b) * si + (unsigned)j * (unsigned)sizeof(T);
+      const T va = *(const T*)(in + (oka ? o : 0u));
+      const T vb = *(const T*)(in + (okb ? o + si : 0u));
+      xa[i] = oka ? va : (T)0;
+      xb[i] = okb ? vb : (T)0;
     }
 #pragma unroll
     for (int i = 0; i < LI; i++) {
@@ -371,69 +387,135 @@ __global__ __launch_bounds__(NT) void k_x_r2c_ct(XArgs a) {
   }
   __syncthreads();
   run_fft_ct<T, true, B, BP, NT>(lds, tw, RL{});
-  cpx<T>* out = (cpx<T>*)a.out;
+  char* out = (char*)a.out + 2 * pair0 * a.so_row * (long long)sizeof(cpx<T>);
+  const unsigned so = (unsigned)(a.so_row * sizeof(cpx<T>));
   const T h = (T)0.5;
+  const bool full = nr == 2 * B;  // uniform: no per-store row guard
   for (int q = threadIdx.x; q < B * a.nxo; q += NT) {
     const int b = fdiv(q, a.dnxo), k = q - b * a.nxo;
     const cpx<T> zk = lds[k * BP + b], zn = lds[(k == 0 ? 0 : N - k) * BP + b];
-    const long long ra = 2 * (pair0 + b), rb = ra + 1;
-    if (ra < a.nrows) out[ra * a.so_row + k] = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h);
-    if (rb < a.nrows) out[rb * a.so_row + k] = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
+    const unsigned o = (unsigned)(2 * b) * so + (unsigned)k * (unsigned)sizeof(cpx<T>);
+    const cpx<T> ya = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h), yb = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
+    if (full) {
+      *(cpx<T>*)(out + o) = ya;
+      *(cpx<T>*)(out + o + so) = yb;
+    } else {
+      if (2 * b < nr) *(cpx<T>*)(out + o) = ya;
+      if (2 * b + 1 < nr) *(cpx<T>*)(out + o + so) = yb;
+    }
   }
 }
 
-template <class T, int B, int NT, class RL>
-__global__ __launch_bounds__(NT) void k_x_c2r_ct(XArgs a) {
-  typedef typename TwT<T>::type TwL;
+template <class T, int B, int NT, class RL, int W, bool TWL>
+__global__ __launch_bounds__(NT, W) void k_x_c2r_ct(XArgs a) {
   constexpr int N = RL_N<RL>::value;
   constexpr int BP = B + 1;
   constexpr int H = N / 2;
   cpx<T>* lds = (cpx<T>*)g_smem;
-  const TwL* tw = (const TwL*)a.tw;
-  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
   const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
-  const cpx<T>* in = (const cpx<T>*)a.in;
+  const int nr = (int)(a.nrows - 2 * pair0 < 2 * B ? a.nrows - 2 * pair0 : 2 * B);
+  const char* in = (const char*)a.in + 2 * pair0 * a.si_row * (long long)sizeof(cpx<T>);
+  const unsigned si = (unsigned)(a.si_row * sizeof(cpx<T>));
   {
-    constexpr int LI = (B * N + NT - 1) / NT;
+    // each input element k <= N/2 is loaded once and gives Z[k] and Z[N-k]
+    // (Z = A + iB of the two Hermitian half spectra)
+    constexpr int K = H + 1;
+    constexpr int LI = (B * K + NT - 1) / NT;
     cpx<T> va[LI], vb[LI];
 #pragma unroll
     for (int i = 0; i < LI; i++) {
       const int q = threadIdx.x + i * NT;
-      const int b = q / N, k = q % N;
-      const long long ra = 2 * (pair0 + b), rb = ra + 1;
-      const int kk = k <= H ? k : N - k;
-      const bool ok = (((B * N) % NT == 0) || q < B * N) && kk < a.nxo;
-      va[i] = (ok && ra < a.nrows) ? in[ra * a.si_row + kk] : mk<T>(0, 0);
-      vb[i] = (ok && rb < a.nrows) ? in[rb * a.si_row + kk] : mk<T>(0, 0);
+      const int b = q / K, k = q % K;
+      const bool ok = (((B * K) % NT == 0) || q < B * K) && k < a.nxo;
+      // out-of-range items load the tile's first element (always valid) and
+      // are zeroed, so no load sits under a branch
+      const bool oka = ok && 2 * b < nr, okb = ok && 2 * b + 1 < nr;
+      const unsigned o = (unsigned)(2 * b) * si + (unsigned)k * (unsigned)sizeof(cpx<T>);
+      const cpx<T> ua = *(const cpx<T>*)(in + (oka ? o : 0u));
+      const cpx<T> ub = *(const cpx<T>*)(in + (okb ? o + si : 0u));
+      va[i] = oka ? ua : mk<T>(0, 0);
+      vb[i] = okb ? ub : mk<T>(0, 0);
     }
 #pragma unroll
     for (int i = 0; i < LI; i++) {
       const int q = threadIdx.x + i * NT;
-      if (((B * N) % NT == 0) || q < B * N) {
-        const int b = q / N, k = q % N;
-        const int kk = k <= H ? k : N - k;
+      if (((B * K) % NT == 0) || q < B * K) {
+        const int b = q / K, k = q % K;
         cpx<T> A = va[i], Bv = vb[i];
-        if (kk == 0 || 2 * kk == N) { A.y = 0; Bv.y = 0; }
-        lds[k * BP + b] = (k == kk) ? mk<T>(A.x - Bv.y, A.y + Bv.x) : mk<T>(A.x + Bv.y, Bv.x - A.y);
+        if (k == 0 || 2 * k == N) { A.y = 0; Bv.y = 0; }
+        // at k = 0 and k = N/2 both forms are the same value: the second
+        // write repeats the first (row 0 stands in for row N)
+        lds[k * BP + b] = mk<T>(A.x - Bv.y, A.y + Bv.x);
+        lds[(k == 0 ? 0 : N - k) * BP + b] = mk<T>(A.x + Bv.y, Bv.x - A.y);
       }
     }
   }
   __syncthreads();
   run_fft_ct<T, false, B, BP, NT>(lds, tw, RL{});
-  T* out = (T*)a.out;
+  char* out = (char*)a.out + 2 * pair0 * a.so_row * (long long)sizeof(T);
+  const unsigned so = (unsigned)(a.so_row * sizeof(T));
+  const bool full = nr == 2 * B;  // uniform: no per-store row guard
   for (int q = threadIdx.x; q < B * N; q += NT) {
     const int b = q / N, j = q % N;
-    const long long ra = 2 * (pair0 + b), rb = ra + 1;
     const cpx<T> z = lds[j * BP + b];
-    if (ra < a.nrows) out[ra * a.so_row + j] = z.x;
-    if (rb < a.nrows) out[rb * a.so_row + j] = z.y;
+    const unsigned o = (unsigned)(2 * b) * so + (unsigned)j * (unsigned)sizeof(T);
+    if (full) {
+      *(T*)(out + o) = z.x;
+      *(T*)(out + o + so) = z.y;
+    } else {
+      if (2 * b < nr) *(T*)(out + o) = z.x;
+      if (2 * b + 1 < nr) *(T*)(out + o + so) = z.y;
+    }
   }
 }
 
-template <class T, bool FWD, int TX, int NT, class RL>
-__global__ __launch_bounds__(NT) void k_col_ct(ColArgs a) {
-  typedef typename TwT<T>::type TwL;
+// Tile of B consecutive columns c0 .. c0+B-1 of the flattened (outer, inner)
+// column index c = o * n_inner + i, element (o, i, row e) at
+// o*so + i*si + e*sl.  The tile base is column c0's address; lane offsets are
+// relative to it and non-negative (so >= n_inner * si).
+template <class T, int B>
+__device__ __forceinline__ ColIO col_io(const ColArgs& a, long long c0, int nb) {
+  ColIO io;
+  const unsigned n_in = (unsigned)a.n_inner;
+  const unsigned o0 = (unsigned)c0 / n_in, i0 = (unsigned)c0 - o0 * n_in;
+  const unsigned c = (unsigned)c0 + threadIdx.x % B;
+  const unsigned o = c / n_in, i = c - o * n_in;
+  constexpr long long E = sizeof(cpx<T>);
+  io.in = (const char*)a.in + ((long long)o0 * a.so_in + (long long)i0 * a.si_in) * E;
+  io.out = (char*)a.out + ((long long)o0 * a.so_out + (long long)i0 * a.si_out) * E;
+  io.lane_in = (unsigned)((((long long)o - o0) * a.so_in + ((long long)i - i0) * a.si_in) * E);
+  io.lane_out = (unsigned)((((long long)o - o0) * a.so_out + ((long long)i - i0) * a.si_out) * E);
+  io.sl_in = (unsigned)(a.sl_in * E);
+  io.sl_out = (unsigned)(a.sl_out * E);
+  io.nb = nb;
+  io.ld_lo = a.ld_lo; io.ld_hi = a.ld_hi; io.ld_shift = a.ld_shift;
+  io.nst = a.nst; io.st_lo = a.st_lo; io.st_shift = a.st_shift;
+  return io;
+}
+
+// C2C column transform with direct global <-> register first/last stages,
+// over tiles of TX consecutive columns of the flattened (outer, inner) index
+// (no tile is partial except the last one).  PR: pruning maps.
+template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL, bool PR>
+__global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
   cpx<T>* lds = (cpx<T>*)g_smem;
+  // the twiddle staging is completed by the barrier at the end of the first stage
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
+  const long long ncol = (long long)a.n_inner * a.n_outer;
+  const long long c0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * TX;
+  const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
+  const ColIO io = col_io<T, TX>(a, c0, nb);
+  run_fft_io<T, FWD, TX, NT, PR>(lds, tw, io, RL{});
+}
+
+// Column transforms through an LDS load/store pass: DCT-I / DST-I
+// extensions, length-fastest ("STRIDE1") views, copies, and views whose
+// offsets do not fit the 32-bit form.
+template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL>
+__global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int o = t / a.tiles;
   const int ti = t - o * a.tiles;
@@ -441,20 +523,6 @@ __global__ __launch_bounds__(NT) void k_col_ct(ColArgs a) {
   const int nb = (int)((long long)a.n_inner - i0 < TX ? (long long)a.n_inner - i0 : TX);
   const cpx<T>* in = (const cpx<T>*)a.in + i0 * a.si_in + (long long)o * a.so_in;
   cpx<T>* out = (cpx<T>*)a.out + i0 * a.si_out + (long long)o * a.so_out;
-  const TwL* tw = (const TwL*)a.tw;
-  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
-  if (a.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast) {
-    // direct global <-> register first/last stages (the twiddle staging above
-    // is completed by the barrier at the end of the first stage)
-    ColIO<T> io;
-    io.in = in; io.out = out;
-    io.si_in = a.si_in; io.sl_in = a.sl_in; io.si_out = a.si_out; io.sl_out = a.sl_out;
-    io.nb = nb;
-    io.ld_lo = a.ld_lo; io.ld_hi = a.ld_hi; io.ld_shift = a.ld_shift;
-    io.nst = a.nst; io.st_lo = a.st_lo; io.st_shift = a.st_shift;
-    run_fft_io<T, FWD, TX, NT>(lds, tw, io, RL{});
-    return;
-  }
   const int off = (a.kind == COL_DST1) ? 1 : 0;
   for (int q = threadIdx.x; q < TX * a.nld; q += NT) {
     int b, e;
@@ -485,7 +553,7 @@ __global__ __launch_bounds__(NT) void k_col_ct(ColArgs a) {
     for (int b = threadIdx.x; b < TX; b += NT) lds[b] = mk<T>(0, 0);
     __syncthreads();
   }
-  run_fft_ct<T, FWD, TX, TX, NT>(lds, tw, RL{});
+  if (a.kind != COL_COPY) run_fft_ct<T, FWD, TX, TX, NT>(lds, tw, RL{});
   for (int q = threadIdx.x; q < TX * a.nst; q += NT) {
     int b, e;
     if (a.st_lenfast) { b = fdiv(q, a.dnst); e = q - b * a.nst; }
@@ -502,25 +570,17 @@ __global__ __launch_bounds__(NT) void k_col_ct(ColArgs a) {
 
 // fused Z-solve: the Z column of a spectrum (nxh, ny, nz), forward FFT,
 // -scale/|k|^2, inverse FFT, in place
-template <class T, int TX, int NT, class RL>
-__global__ __launch_bounds__(NT) void k_zsolve_ct(ColArgs a, double scale, int nxh, int ny) {
-  typedef typename TwT<T>::type TwL;
+template <class T, int TX, int NT, class RL, int W, bool TWL>
+__global__ __launch_bounds__(NT, W) void k_zsolve_ct(ColArgs a, double scale, int nxh, int ny) {
   cpx<T>* lds = (cpx<T>*)g_smem;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const long long i0 = (long long)t * TX;
-  const int nb = (int)((long long)a.n_inner - i0 < TX ? (long long)a.n_inner - i0 : TX);
-  const TwL* tw = (const TwL*)a.tw;
-  if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
-  ColIO<T> io;
-  io.in = (const cpx<T>*)a.in + i0;
-  io.out = (cpx<T>*)a.out + i0;
-  io.si_in = 1; io.sl_in = a.sl_in; io.si_out = 1; io.sl_out = a.sl_out;
-  io.nb = nb;
-  io.ld_lo = a.L.n; io.ld_hi = a.L.n; io.ld_shift = 0;
-  io.nst = a.L.n; io.st_lo = a.L.n; io.st_shift = 0;
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
+  const long long ncol = a.n_inner;
+  const long long c0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * TX;
+  const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
+  const ColIO io = col_io<T, TX>(a, c0, nb);
   SolveCtx sc;
-  sc.i0 = i0; sc.nxh = nxh; sc.ny = ny; sc.nz = a.L.n; sc.scale = scale;
-  run_fft_solve<T, TX, NT>(lds, tw, io, sc, RL{});
+  sc.nxh = nxh; sc.ny = ny; sc.nz = a.L.n; sc.scale = scale;
+  run_fft_solve<T, TX, NT>(lds, tw, io, c0, sc, RL{});
 }
 
 // ---------------------------------------------------------------------------
@@ -769,17 +829,34 @@ static void dispatch_nt(int nt, const A& a, unsigned g, size_t b, hipStream_t s)
 }
 
 // ---- compile-time specialised lengths ----------------------------------
-constexpr int kEPTct = sizeof(real_t) == 4 ? 32 : 16;  // complex values per thread
+// Tuning constants of the specialised kernels (compile-time): complex values
+// per thread and bytes of adjacent columns per row of a column tile.
+#ifndef P3DFFT_CT_EPT
+#define P3DFFT_CT_EPT (sizeof(real_t) == 4 ? 32 : 16)
+#endif
+#ifndef P3DFFT_CT_COL_BYTES
+#define P3DFFT_CT_COL_BYTES 128
+#endif
+#ifndef P3DFFT_CT_NT_MAX
+#define P3DFFT_CT_NT_MAX 256
+#endif
+constexpr int kEPTct = P3DFFT_CT_EPT;
 constexpr int pow2floor(int x) { int p = 1; while (p * 2 <= x) p *= 2; return p; }
 constexpr int ct_col_tx(int N) {
-  int tx = 128 / (int)sizeof(cplx_t);
+  int tx = P3DFFT_CT_COL_BYTES / (int)sizeof(cplx_t);
   int cap = pow2floor(256 * kEPTct / N);
   if (cap < 1) cap = 1;
   return tx < cap ? tx : cap;
 }
 constexpr int ct_nt(int elems) {
   int nt = (elems / kEPTct + 63) / 64 * 64;
-  return nt < 64 ? 64 : (nt > 256 ? 256 : nt);
+  return nt < 64 ? 64 : (nt > P3DFFT_CT_NT_MAX ? P3DFFT_CT_NT_MAX : nt);
+}
+constexpr int ct_waves(long long lds_bytes, int nt) {
+  long long wg = lds_bytes > 0 ? (160 * 1024) / lds_bytes : 8;
+  if (wg < 1) wg = 1;
+  const long long w = wg * (nt / 64) / 4;
+  return w < 1 ? 1 : (w > 4 ? 4 : (int)w);  // at least 128 registers per lane
 }
 constexpr int ct_x_pairs(int N) {
   int b = 16;
@@ -787,10 +864,18 @@ constexpr int ct_x_pairs(int N) {
   return b;
 }
 
+// Twiddles of a specialised length sit in LDS behind the data tile when the
+// two fit in this many bytes (two workgroups per CU at the largest tiles).
+constexpr long long kCtLdsTw = 80 * 1024;
+constexpr long long round16(long long x) { return (x + 15) / 16 * 16; }
+
 struct CtInfo {
   int TX, NTC, BX, NTX, nr;
   int radix[kMaxStages];
+  bool twl_col, twl_x;  // twiddles staged in LDS (column / X kernels)
+  long long twb;        // twiddle table bytes
   void (*col)(bool, const ColArgs&, unsigned, size_t, hipStream_t);
+  void (*colio)(bool, bool, const ColArgs&, unsigned, size_t, hipStream_t);
   void (*xr2c)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*xc2r)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*zsolve)(const ColArgs&, double, int, int, unsigned, size_t, hipStream_t);
@@ -804,31 +889,50 @@ struct CtLen {
   static constexpr int NTC = ct_nt(TX * N);
   static constexpr int BX = ct_x_pairs(N);
   static constexpr int NTX = ct_nt(BX * N);
+  static constexpr long long TWB = (long long)TwCount<Rs...>::value() * sizeof(Tw);
+  static constexpr long long DC = round16((long long)TX * N * sizeof(cplx_t));
+  static constexpr long long DX = round16((long long)(BX + 1) * N * sizeof(cplx_t));
+  static constexpr bool TWLC = TWB > 0 && DC + TWB <= kCtLdsTw;
+  static constexpr bool TWLX = TWB > 0 && DX + TWB <= kCtLdsTw;
+  // waves per SIMD the LDS footprint allows: the register budget of each kernel
+  static constexpr int WC = ct_waves(DC + (TWLC ? TWB : 0), NTC);
+  static constexpr int WX = ct_waves(DX + (TWLX ? TWB : 0), NTX);
+  template <class K>
+  static void go(K k, unsigned g, int nt, size_t bytes, hipStream_t s, const ColArgs& a) {
+    static bool once = (allow_big_lds(k), true);
+    (void)once;
+    k<<<g, nt, bytes, s>>>(a);
+  }
+  template <class K>
+  static void gox(K k, unsigned g, int nt, size_t bytes, hipStream_t s, const XArgs& a) {
+    static bool once = (allow_big_lds(k), true);
+    (void)once;
+    k<<<g, nt, bytes, s>>>(a);
+  }
   static void col(bool fwd, const ColArgs& a, unsigned g, size_t bytes, hipStream_t s) {
+    if (fwd) go(k_col_ct<real_t, true, TX, NTC, RL, WC, TWLC>, g, NTC, bytes, s, a);
+    else go(k_col_ct<real_t, false, TX, NTC, RL, WC, TWLC>, g, NTC, bytes, s, a);
+  }
+  static void colio(bool fwd, bool pr, const ColArgs& a, unsigned g, size_t bytes, hipStream_t s) {
     if (fwd) {
-      static bool once = (allow_big_lds(k_col_ct<real_t, true, TX, NTC, RL>), true);
-      (void)once;
-      k_col_ct<real_t, true, TX, NTC, RL><<<g, NTC, bytes, s>>>(a);
+      if (pr) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, true>, g, NTC, bytes, s, a);
+      else go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, false>, g, NTC, bytes, s, a);
     } else {
-      static bool once = (allow_big_lds(k_col_ct<real_t, false, TX, NTC, RL>), true);
-      (void)once;
-      k_col_ct<real_t, false, TX, NTC, RL><<<g, NTC, bytes, s>>>(a);
+      if (pr) go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, true>, g, NTC, bytes, s, a);
+      else go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, false>, g, NTC, bytes, s, a);
     }
   }
   static void xr2c(const XArgs& a, unsigned g, size_t bytes, hipStream_t s) {
-    static bool once = (allow_big_lds(k_x_r2c_ct<real_t, BX, NTX, RL>), true);
-    (void)once;
-    k_x_r2c_ct<real_t, BX, NTX, RL><<<g, NTX, bytes, s>>>(a);
+    gox(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX>, g, NTX, bytes, s, a);
   }
   static void xc2r(const XArgs& a, unsigned g, size_t bytes, hipStream_t s) {
-    static bool once = (allow_big_lds(k_x_c2r_ct<real_t, BX, NTX, RL>), true);
-    (void)once;
-    k_x_c2r_ct<real_t, BX, NTX, RL><<<g, NTX, bytes, s>>>(a);
+    gox(k_x_c2r_ct<real_t, BX, NTX, RL, WX, TWLX>, g, NTX, bytes, s, a);
   }
   static void zsolve(const ColArgs& a, double scale, int nxh, int ny, unsigned g, size_t bytes, hipStream_t s) {
-    static bool once = (allow_big_lds(k_zsolve_ct<real_t, TX, NTC, RL>), true);
+    auto k = k_zsolve_ct<real_t, TX, NTC, RL, WC, TWLC>;
+    static bool once = (allow_big_lds(k), true);
     (void)once;
-    k_zsolve_ct<real_t, TX, NTC, RL><<<g, NTC, bytes, s>>>(a, scale, nxh, ny);
+    k<<<g, NTC, bytes, s>>>(a, scale, nxh, ny);
   }
   static CtInfo info() {
     CtInfo c;
@@ -837,7 +941,8 @@ struct CtLen {
     const int rs[] = {Rs...};
     c.nr = (int)sizeof...(Rs);
     for (int i = 0; i < c.nr; i++) c.radix[i] = rs[i];
-    c.col = &col; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve;
+    c.twl_col = TWLC; c.twl_x = TWLX; c.twb = TWB;
+    c.col = &col; c.colio = &colio; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve;
     return c;
   }
 };
@@ -889,18 +994,28 @@ static void x_setup(XArgs& a, const Len1D& L, long long nrows, int nx, size_t* b
   *nwg = (unsigned)g;
 }
 
-static void x_setup_ct(XArgs& a, const Len1D& L, const CtInfo& c, long long nrows, int nx, size_t* bytes,
-                       unsigned* nwg) {
+// LDS bytes of a specialised tile of `data` bytes; sets the twiddle placement
+static size_t ct_lds(size_t data, bool twl, const CtInfo& c, int* tw_lds, int* tw_off) {
+  const size_t d16 = (size_t)round16((long long)data);
+  *tw_lds = twl ? 1 : 0;
+  *tw_off = twl ? (int)d16 : 0;
+  return twl ? d16 + (size_t)c.twb : data;
+}
+
+// false when the 32-bit tile offsets of the specialised X kernels do not fit
+static bool x_setup_ct(XArgs& a, const Len1D& L, const CtInfo& c, long long nrows, long long si_row,
+                       long long so_row, size_t* bytes, unsigned* nwg) {
+  const long long span = (2LL * c.BX + 1) * std::max(si_row, so_row) * (long long)sizeof(cplx_t);
+  if (span >= (1LL << 32)) return false;
   const long long pairs = (nrows + 1) / 2;
   a.B = c.BX;
   a.L = L.d;
   a.tw = L.tw;
-  const size_t data = (size_t)(c.BX + 1) * nx * sizeof(cplx_t);
-  *bytes = lds_layout(data, L, &a.tw_lds, &a.tw_off);
-  if (*bytes > 80 * 1024) { a.tw_lds = 0; a.tw_off = 0; *bytes = data; }
+  *bytes = ct_lds((size_t)(c.BX + 1) * L.d.n * sizeof(cplx_t), c.twl_x, c, &a.tw_lds, &a.tw_off);
   const long long g = (pairs + c.BX - 1) / c.BX;
   if (g > 0x7fffffffLL) throw Error{"grid too large", -1};
   *nwg = (unsigned)g;
+  return true;
 }
 
 void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_row, long long so_row, int nx,
@@ -913,10 +1028,11 @@ void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_r
   a.nxo = nxo; a.dnxo = make_fdiv(nxo);
   if (const CtInfo* c = g_use_ct ? find_ct(L) : nullptr) {
     size_t bytes; unsigned nwg;
-    x_setup_ct(a, L, *c, nrows, nx, &bytes, &nwg);
-    c->xr2c(a, nwg, bytes, st);
-    check_hip(hipGetLastError(), "k_x_r2c_ct launch");
-    return;
+    if (x_setup_ct(a, L, *c, nrows, si_row, so_row, &bytes, &nwg)) {
+      c->xr2c(a, nwg, bytes, st);
+      check_hip(hipGetLastError(), "k_x_r2c_ct launch");
+      return;
+    }
   }
   size_t bytes; int nt; unsigned nwg;
   x_setup(a, L, nrows, nx, &bytes, &nt, &nwg);
@@ -934,15 +1050,27 @@ void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_r
   a.nxo = nxi; a.dnxo = make_fdiv(nxi > 0 ? nxi : 1);
   if (const CtInfo* c = g_use_ct ? find_ct(L) : nullptr) {
     size_t bytes; unsigned nwg;
-    x_setup_ct(a, L, *c, nrows, nx, &bytes, &nwg);
-    c->xc2r(a, nwg, bytes, st);
-    check_hip(hipGetLastError(), "k_x_c2r_ct launch");
-    return;
+    if (x_setup_ct(a, L, *c, nrows, si_row, so_row, &bytes, &nwg)) {
+      c->xc2r(a, nwg, bytes, st);
+      check_hip(hipGetLastError(), "k_x_c2r_ct launch");
+      return;
+    }
   }
   size_t bytes; int nt; unsigned nwg;
   x_setup(a, L, nrows, nx, &bytes, &nt, &nwg);
   dispatch_nt<GoXc2r>(nt, a, nwg, bytes, st);
   check_hip(hipGetLastError(), "k_x_c2r launch");
+}
+
+// The byte offsets of a column tile relative to its first column stay below
+// 2^32: lanes span at most TX/n_inner + 2 outer indices and n_inner inner
+// ones, plus `rows` rows.
+static bool col_fits32(const ColSpec& c, int TX, int rows) {
+  const long long nout = TX / std::max(c.n_inner, 1) + 2;
+  auto span = [&](long long si, long long sl, long long so) {
+    return (nout * so + (long long)c.n_inner * si + (long long)rows * sl + 1) * (long long)sizeof(cplx_t);
+  };
+  return span(c.si_in, c.sl_in, c.so_in) < (1LL << 32) && span(c.si_out, c.sl_out, c.so_out) < (1LL << 32);
 }
 
 void launch_col(const ColSpec& c, hipStream_t st) {
@@ -982,14 +1110,21 @@ void launch_col(const ColSpec& c, hipStream_t st) {
     a.TX = TX;
     a.dTX = make_fdiv(TX);
     a.tiles = (c.n_inner + TX - 1) / TX;
-    const long long nwg = (long long)a.tiles * c.n_outer;
-    if (nwg > 0x7fffffffLL) throw Error{"grid too large", -1};
     a.L = L->d;
     a.tw = L->tw;
-    const size_t data = (size_t)TX * rows * sizeof(cplx_t);
-    size_t bytes = lds_layout(data, *L, &a.tw_lds, &a.tw_off);
-    if (bytes > 80 * 1024) { a.tw_lds = 0; a.tw_off = 0; bytes = data; }
+    const size_t bytes = ct_lds((size_t)TX * rows * sizeof(cplx_t), ct->twl_col, *ct, &a.tw_lds, &a.tw_off);
     if (bytes > kLdsMax) throw Error{"column transform does not fit in LDS", -1};
+    const long long ncol = (long long)c.n_inner * c.n_outer;
+    if (ncol > 0x7fffffffLL) throw Error{"grid too large", -1};
+    if (c.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast && col_fits32(c, TX, rows)) {
+      // identity row maps: no pruning on either side
+      const bool pr = !(c.nld == nfft && c.ld_lo >= nfft && c.nst == nfft && c.st_lo >= nfft);
+      ct->colio(fwd, pr, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
+      check_hip(hipGetLastError(), "k_colio_ct launch");
+      return;
+    }
+    const long long nwg = (long long)a.tiles * c.n_outer;
+    if (nwg > 0x7fffffffLL) throw Error{"grid too large", -1};
     ct->col(fwd, a, (unsigned)nwg, bytes, st);
     check_hip(hipGetLastError(), "k_col_ct launch");
     return;
@@ -1040,9 +1175,8 @@ bool launch_zsolve(cplx_t* spec, int nxh, int ny, int nz, double scale, hipStrea
   a.tiles = (int)((ninner + TX - 1) / TX);
   a.L = L.d;
   a.tw = L.tw;
-  const size_t data = (size_t)TX * nz * sizeof(cplx_t);
-  size_t bytes = lds_layout(data, L, &a.tw_lds, &a.tw_off);
-  if (bytes > 80 * 1024) { a.tw_lds = 0; a.tw_off = 0; bytes = data; }
+  if ((ninner + (long long)nz * ninner + TX) * (long long)sizeof(cplx_t) >= (1LL << 32)) return false;
+  const size_t bytes = ct_lds((size_t)TX * nz * sizeof(cplx_t), ct->twl_col, *ct, &a.tw_lds, &a.tw_off);
   if (bytes > kLdsMax) return false;
   ct->zsolve(a, scale, nxh, ny, (unsigned)a.tiles, bytes, st);
   check_hip(hipGetLastError(), "k_zsolve_ct launch");

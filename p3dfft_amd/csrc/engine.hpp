@@ -85,6 +85,7 @@ void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long 
 // use the compile-time specialised kernels where a length has them (default on)
 void set_specialised(bool on);
 
+
 // out[i] = in[i] over n16 16-byte vectors
 void launch_copy16(const void* in, void* out, long long n16, hipStream_t st);
 

@@ -80,44 +80,68 @@ __device__ __forceinline__ void run_fft_ct(cpx<T>* lds, const typename TwT<T>::t
 // ---------------------------------------------------------------------------
 // Column transforms with the first stage fed straight from global memory and
 // the last stage stored straight to global memory (one LDS round trip fewer
-// at each end, and every thread keeps IT*R independent loads in flight).
-// The view is [len][inner] with the inner index contiguous; rows are remapped
-// on load (pruning zero-fill) and on store (pruning selection).
+// at each end; every thread keeps IT*R independent loads in flight).
+// Addresses are a wave-uniform 64-bit tile base plus 32-bit per-lane byte
+// offsets, so an element access costs one 32-bit add.  With PR the rows are
+// remapped on load (pruning zero-fill) and on store (pruning selection);
+// without it both maps are the identity.
 // ---------------------------------------------------------------------------
-template <class T>
 struct ColIO {
-  const cpx<T>* in;
-  cpx<T>* out;
-  long long si_in, sl_in, si_out, sl_out;
+  const char* in;             // input tile base
+  char* out;                  // output tile base
+  unsigned lane_in, lane_out; // this lane's column, bytes from the tile bases
+  unsigned sl_in, sl_out;     // row strides, bytes
   int nb;                     // valid columns of this tile
-  int ld_lo, ld_hi, ld_shift; // LDS row e <- input row e (e<ld_lo), e-ld_shift (e>=ld_hi), else 0
-  int nst, st_lo, st_shift;   // output row e <- FFT row e (e<st_lo) or e+st_shift
+  int ld_lo, ld_hi, ld_shift; // PR: LDS row e <- input row e (e<ld_lo), e-ld_shift (e>=ld_hi), else 0
+  int nst, st_lo, st_shift;   // PR: output row e <- FFT row e (e<st_lo) or e+st_shift
 };
 
-template <class T, int R, bool FWD, int N, int B, int NT>
-__device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO<T>& io) {
-  constexpr int nR = N / R;
-  constexpr int total = B * nR;
-  constexpr int IT = (total + NT - 1) / NT;
-  cpx<T> v[IT][R];
+template <int R, int N, int B, int NT>
+struct Stage0 {
+  static constexpr int nR = N / R;
+  static constexpr int total = B * nR;
+  static constexpr int IT = (total + NT - 1) / NT;
+};
+
+template <class T>
+__device__ __forceinline__ cpx<T> ld_at(const char* base, unsigned off) {
+  return *(const cpx<T>*)(base + off);
+}
+
+// first stage: loads of one tile into registers, butterflies (no twiddles),
+// results into the LDS tile
+template <class T, int R, bool FWD, int N, int B, int NT, bool PR>
+__device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
+  static_assert(NT % B == 0, "one column per thread");
+  typedef Stage0<R, N, B, NT> S;
+  const bool valid = (int)(threadIdx.x % B) < io.nb;
+  cpx<T> v[S::IT][R];
 #pragma unroll
-  for (int m = 0; m < IT; m++) {
+  for (int m = 0; m < S::IT; m++) {
     const int it = threadIdx.x + m * NT;
-    if ((total % NT) == 0 || it < total) {
-      const int j = it / B, b = it % B;
+    if ((S::total % NT) == 0 || it < S::total) {
+      const int j = it / B;
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        const int e = j + r * nR;
-        const int src = e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1);
-        v[m][r] = (b < io.nb && src >= 0) ? io.in[(long long)b * io.si_in + (long long)src * io.sl_in]
-                                          : mk<T>(0, 0);
+        const int e = j + r * S::nR;
+        // items outside the tile load its first element (always valid) and
+        // are zeroed, so no load sits under a branch
+        if constexpr (PR) {
+          const int src = e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1);
+          const bool ok = valid && src >= 0;
+          const cpx<T> u = ld_at<T>(io.in, ok ? io.lane_in + (unsigned)src * io.sl_in : 0u);
+          v[m][r] = ok ? u : mk<T>(0, 0);
+        } else {
+          const cpx<T> u = ld_at<T>(io.in, valid ? io.lane_in + (unsigned)e * io.sl_in : 0u);
+          v[m][r] = valid ? u : mk<T>(0, 0);
+        }
       }
     }
   }
 #pragma unroll
-  for (int m = 0; m < IT; m++) {
+  for (int m = 0; m < S::IT; m++) {
     const int it = threadIdx.x + m * NT;
-    if ((total % NT) == 0 || it < total) {
+    if ((S::total % NT) == 0 || it < S::total) {
       const int j = it / B, b = it % B;
       dft<R, FWD>(v[m]);
 #pragma unroll
@@ -128,12 +152,14 @@ __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO<T>& io) {
 }
 
 // last stage: Ns * R == N, so every butterfly index j < Ns and k == j
-template <class T, int R, bool FWD, int N, int B, int NT, int Ns, int Off>
+template <class T, int R, bool FWD, int N, int B, int NT, int Ns, int Off, bool PR>
 __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename TwT<T>::type* tw,
-                                              const ColIO<T>& io) {
+                                              const ColIO& io) {
   static_assert(Ns * R == N, "last stage");
+  static_assert(NT % B == 0, "one column per thread");
   constexpr int total = B * Ns;
   constexpr int IT = (total + NT - 1) / NT;
+  const bool valid = (int)(threadIdx.x % B) < io.nb;
 #pragma unroll
   for (int m = 0; m < IT; m++) {
     const int it = threadIdx.x + m * NT;
@@ -146,57 +172,60 @@ __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename 
 #pragma unroll
       for (int r = 1; r < R; r++) v[r] = twmul<FWD>(v[r], w[r - 1]);
       dft<R, FWD>(v);
-      if (b < io.nb) {
+      if (valid) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
           const int p = j + r * Ns;
-          const int e = p < io.st_lo ? p : p - io.st_shift;
-          if (p < io.st_lo || (p >= io.st_lo + io.st_shift && e < io.nst))
-            io.out[(long long)b * io.si_out + (long long)e * io.sl_out] = v[r];
+          if constexpr (PR) {
+            const int e = p < io.st_lo ? p : p - io.st_shift;
+            if (p < io.st_lo || (p >= io.st_lo + io.st_shift && e < io.nst))
+              *(cpx<T>*)(io.out + (io.lane_out + (unsigned)e * io.sl_out)) = v[r];
+          } else {
+            *(cpx<T>*)(io.out + (io.lane_out + (unsigned)p * io.sl_out)) = v[r];
+          }
         }
       }
     }
   }
 }
 
-template <class T, bool FWD, int N, int B, int NT, int Ns, int Off, int R, int... Rest>
-__device__ __forceinline__ void mid_and_last(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO<T>& io) {
+template <class T, bool FWD, int N, int B, int NT, int Ns, int Off, bool PR, int R, int... Rest>
+__device__ __forceinline__ void mid_and_last(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO& io) {
   if constexpr (sizeof...(Rest) == 0) {
-    stageL_global<T, R, FWD, N, B, NT, Ns, Off>(lds, tw, io);
+    stageL_global<T, R, FWD, N, B, NT, Ns, Off, PR>(lds, tw, io);
   } else {
     stage_ct<T, R, FWD, N, B, B, NT, Ns, Off>(lds, tw);
-    mid_and_last<T, FWD, N, B, NT, Ns * R, (Ns > 1 ? Off + Ns * (R - 1) : Off), Rest...>(lds, tw, io);
+    mid_and_last<T, FWD, N, B, NT, Ns * R, (Ns > 1 ? Off + Ns * (R - 1) : Off), PR, Rest...>(lds, tw, io);
   }
 }
 
-template <class T, bool FWD, int B, int NT, int R0, int... Rs>
-__device__ __forceinline__ void run_fft_io(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO<T>& io,
+template <class T, bool FWD, int B, int NT, bool PR, int R0, int... Rs>
+__device__ __forceinline__ void run_fft_io(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO& io,
                                            RList<R0, Rs...>) {
   static_assert(sizeof...(Rs) >= 1, "at least two stages");
   constexpr int N = R0 * RProd<Rs...>::value;
-  stage0_global<T, R0, FWD, N, B, NT>(lds, io);
-  mid_and_last<T, FWD, N, B, NT, R0, 0, Rs...>(lds, tw, io);
+  stage0_global<T, R0, FWD, N, B, NT, PR>(lds, io);
+  mid_and_last<T, FWD, N, B, NT, R0, 0, PR, Rs...>(lds, tw, io);
 }
 
 // Spectral solve along the column: forward FFT, per-mode multiply by
-// -scale/|k|^2 (k=0 -> 0), inverse FFT, all in one LDS tile.  The inner index
-// of column b is i0 + b = kx + nxh * ky_index; the row is the kz index.
+// -scale/|k|^2 (k=0 -> 0), inverse FFT, all in one LDS tile.  Column i0 + b
+// of the tile is kx + nxh * ky_index; the row is the kz index.
 struct SolveCtx {
-  long long i0;
   int nxh, ny, nz;
   double scale;
 };
 
 template <class T, int B, int NT, int R0, int... Rs>
-__device__ __forceinline__ void run_fft_solve(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO<T>& io,
-                                              const SolveCtx& sc, RList<R0, Rs...>) {
+__device__ __forceinline__ void run_fft_solve(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO& io,
+                                              long long i0, const SolveCtx& sc, RList<R0, Rs...>) {
   static_assert(sizeof...(Rs) >= 1, "at least two stages");
   constexpr int N = R0 * RProd<Rs...>::value;
-  stage0_global<T, R0, true, N, B, NT>(lds, io);
+  stage0_global<T, R0, true, N, B, NT, false>(lds, io);
   stages_ct<T, true, N, B, B, NT, R0, 0, Rs...>(lds, tw);
   for (int q = threadIdx.x; q < B * N; q += NT) {
     const int e = q / B, b = q % B;
-    const long long i = sc.i0 + b;
+    const long long i = i0 + b;
     const int kx = (int)(i % sc.nxh);
     const int y = (int)(i / sc.nxh);
     const double ky = y <= sc.ny / 2 ? y : y - sc.ny;
@@ -207,7 +236,7 @@ __device__ __forceinline__ void run_fft_solve(cpx<T>* lds, const typename TwT<T>
     lds[q] = mk<T>((T)(v.x * f), (T)(v.y * f));
   }
   __syncthreads();
-  mid_and_last<T, false, N, B, NT, 1, 0, R0, Rs...>(lds, tw, io);
+  mid_and_last<T, false, N, B, NT, 1, 0, false, R0, Rs...>(lds, tw, io);
 }
 
 // Twiddle table entries used by a radix list (all stages after the first).
