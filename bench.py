@@ -32,8 +32,8 @@ import p3dfft_amd  # noqa: E402
 
 HBM_PEAK = 8.0e12
 METRIC = "fwd+bwd 3D FFT time & % HBM roofline at 512^3; batched/served solves/s, p99"
-SLOT_NAMES = {5: "x_r2c (k_x_r2c)", 7: "y_fwd (k_col)", 8: "z_fwd (k_col)", 9: "z_bwd (k_col)",
-              10: "y_bwd (k_col)", 12: "x_c2r (k_x_c2r)"}
+SLOT_NAMES = {5: "x_r2c (k_x_r2c_ct)", 7: "y_fwd (k_colio_ct)", 8: "z_fwd (k_colio_ct)",
+              9: "z_bwd (k_colio_ct)", 10: "y_bwd (k_colio_ct)", 12: "x_c2r (k_x_c2r_ct)"}
 
 
 def modes_field(n, seed):
@@ -57,6 +57,21 @@ def slot_bytes(slot, n, s):
     if slot in (5, 12):
         return nr * s + nc * 2 * s
     return 2 * nc * 2 * s
+
+
+PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_512cubed_fp32.md")
+
+
+def pmc_traffic(stage):
+    """HBM bytes per launch of a stage kernel at 512^3 FP32, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    if not os.path.exists(PMC_TRAFFIC):
+        return None
+    for line in open(PMC_TRAFFIC):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == stage and cells[4].isdigit():
+            return int(cells[4])
+    return None
 
 
 def log(msg):
@@ -186,7 +201,10 @@ def serve_leg(n_req, slots, load_frac=0.7):
         lat_q, _, _ = S.run(order, arr, sequential=True)
     finally:
         S.close()
-    # the plain FP64 GPU path, one solve at a time, on the same requests
+    # the plain FP64 GPU path, one solve at a time, on the same requests:
+    # (a) the fused solve entry point with a synchronise after each request,
+    # (b) the reference's entry points per request (setup, ftran_r2c, the
+    # multiply, btran_c2r, clean), as a caller of its single global plan
     S8 = serve.PoissonServer(shapes, precision=8, slots=1)
     try:
         S8.fill()
@@ -194,12 +212,19 @@ def serve_leg(n_req, slots, load_frac=0.7):
         S8.fill()
         _, mk_seq64, _ = S8.run(order, np.zeros(n_req), sequential=True)
         thr_seq64 = n_req / mk_seq64
+        S8.fill()
+        thr_api64 = n_req / S8.run_api(order)
     finally:
         S8.close()
     return {"requests": n_req, "slots": slots, "solves_per_s": thr_served, "one_at_a_time_solves_per_s": thr_seq,
             "speedup_vs_one_at_a_time": thr_served / thr_seq,
-            "one_at_a_time_plain_fp64_solves_per_s": thr_seq64,
-            "speedup_vs_one_at_a_time_plain_fp64": thr_served / thr_seq64,
+            "one_at_a_time_fp64_solves_per_s": thr_seq64,
+            "speedup_vs_one_at_a_time_fp64": thr_served / thr_seq64,
+            "one_at_a_time_plain_fp64_api_solves_per_s": thr_api64,
+            "speedup_vs_one_at_a_time_plain_fp64_api": thr_served / thr_api64,
+            "baselines": "one_at_a_time: the same FP32 fused solve, synchronised after each request; "
+                         "fp64: the FP64 fused solve likewise; plain_fp64_api: per request p3dfft_setup, "
+                         "ftran_r2c_dev 'fft', p3dfft_poisson_multiply, btran_c2r_dev 'tff', p3dfft_clean (FP64)",
             "latency_load": "exponential inter-arrival at %.0f%% of served capacity (%.0f req/s)"
                             % (100 * load_frac, load_frac * thr_served),
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
@@ -275,7 +300,10 @@ def main():
                                "fields resident in HBM" % n, "grid": [n, n, n], "op": ["fft", "tff"],
                    "xy_chunk_mib": 0 if args.chunk_mb is None else args.chunk_mb},
         "roofline": {"bound": "hbm", "kernel": SLOT_NAMES[dom_slot], "achieved": achieved / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": pmc_traffic(SLOT_NAMES[dom_slot].split()[0]) if n == 512 else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, calibrated on a copy kernel: "
+                                       "profiles/pmc_traffic_512cubed_fp32.md",
                      "bytes_per_launch": slot_bytes(dom_slot, n, s), "avg_launch_ms": dom_avg * 1e3},
         "path": {"compulsory_bytes": 2 * B, "frac_of_8TBps": 2 * B / t / HBM_PEAK,
                  "copy_kernel_GBps": cpk / 1e9, "frac_of_copy_peak": 2 * B / t / cpk,

@@ -144,6 +144,9 @@ int p3dfft_poisson_solve(int nx, int ny, int nz, p3dfft_real *field, void *work,
 /* 1 (default): Z forward, multiply and Z inverse fused in one kernel; 0: the
  * composition of the forward transform, a multiply pass and the backward one */
 void p3dfft_poisson_set_fused(int on);
+/* The multiply step alone, on a spectrum (nx/2+1, ny, nz) from
+ * p3dfft_ftran_r2c_dev: times -1/(|k|^2 nx ny nz), k = 0 set to 0. */
+int p3dfft_poisson_multiply(p3dfft_real *spec, int nx, int ny, int nz, void *stream);
 /* build the plan and twiddle table of a transform length ahead of time */
 int p3dfft_amd_prepare_length(int n);
 /* i.i.d. N(0,1) values from a counter-based generator keyed by seed */

@@ -166,6 +166,21 @@ int p3dfft_poisson_solve(int nx, int ny, int nz, p3amd::real_t* field, void* wor
   }
 }
 
+// The per-mode multiply of a solve on its own: spectrum (nx/2+1, ny, nz) in
+// HBM times -1/(|k|^2 nx ny nz), k=0 -> 0, for callers that compose a solve
+// from p3dfft_ftran_r2c_dev and p3dfft_btran_c2r_dev.  Asynchronous.
+int p3dfft_poisson_multiply(p3amd::real_t* spec, int nx, int ny, int nz, void* stream) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) {
+    set_error("p3dfft_poisson_multiply: invalid grid");
+    return -1;
+  }
+  const int nxhp = nx / 2 + 1;
+  const long long n = (long long)nxhp * ny * nz;
+  k_poisson<real_t><<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      (cplx_t*)spec, nxhp, ny, nz, 1.0 / ((double)nx * ny * nz));
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // Fill n values with i.i.d. N(0,1) from a counter-based generator keyed by seed.
 int p3dfft_amd_fill_normal(p3amd::real_t* p, long long n, unsigned long long seed, void* stream) {
   if (n <= 0) return 0;

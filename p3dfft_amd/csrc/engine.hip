@@ -844,7 +844,7 @@ constexpr int kEPTct = P3DFFT_CT_EPT;
 constexpr int pow2floor(int x) { int p = 1; while (p * 2 <= x) p *= 2; return p; }
 constexpr int ct_col_tx(int N) {
   int tx = P3DFFT_CT_COL_BYTES / (int)sizeof(cplx_t);
-  int cap = pow2floor(256 * kEPTct / N);
+  int cap = pow2floor(P3DFFT_CT_NT_MAX * kEPTct / N);
   if (cap < 1) cap = 1;
   return tx < cap ? tx : cap;
 }
@@ -1117,8 +1117,10 @@ void launch_col(const ColSpec& c, hipStream_t st) {
     const long long ncol = (long long)c.n_inner * c.n_outer;
     if (ncol > 0x7fffffffLL) throw Error{"grid too large", -1};
     if (c.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast && col_fits32(c, TX, rows)) {
-      // identity row maps: no pruning on either side
-      const bool pr = !(c.nld == nfft && c.ld_lo >= nfft && c.nst == nfft && c.st_lo >= nfft);
+      // identity row maps (no zero gap, no shift, all rows): no pruning
+      const bool ld_id = c.nld == nfft && (c.ld_lo >= nfft || (c.ld_hi <= c.ld_lo && c.ld_shift == 0));
+      const bool st_id = c.nst == nfft && (c.st_lo >= nfft || c.st_shift == 0);
+      const bool pr = !(ld_id && st_id);
       ct->colio(fwd, pr, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
       check_hip(hipGetLastError(), "k_colio_ct launch");
       return;

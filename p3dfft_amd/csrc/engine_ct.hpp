@@ -108,6 +108,11 @@ __device__ __forceinline__ cpx<T> ld_at(const char* base, unsigned off) {
   return *(const cpx<T>*)(base + off);
 }
 
+template <class T>
+__device__ __forceinline__ void st_at(char* base, unsigned off, cpx<T> v) {
+  *(cpx<T>*)(base + off) = v;
+}
+
 // first stage: loads of one tile into registers, butterflies (no twiddles),
 // results into the LDS tile
 template <class T, int R, bool FWD, int N, int B, int NT, bool PR>
@@ -179,9 +184,9 @@ __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename 
           if constexpr (PR) {
             const int e = p < io.st_lo ? p : p - io.st_shift;
             if (p < io.st_lo || (p >= io.st_lo + io.st_shift && e < io.nst))
-              *(cpx<T>*)(io.out + (io.lane_out + (unsigned)e * io.sl_out)) = v[r];
+              st_at<T>(io.out, io.lane_out + (unsigned)e * io.sl_out, v[r]);
           } else {
-            *(cpx<T>*)(io.out + (io.lane_out + (unsigned)p * io.sl_out)) = v[r];
+            st_at<T>(io.out, io.lane_out + (unsigned)p * io.sl_out, v[r]);
           }
         }
       }

@@ -36,6 +36,8 @@ def _bind(L):
     L.p3dfft_amd_event_query.argtypes = [vp]
     L.p3dfft_amd_event_query.restype = ctypes.c_int
     L.p3dfft_poisson_set_fused.argtypes = [ctypes.c_int]
+    L.p3dfft_poisson_multiply.argtypes = [vp] + [ctypes.c_int] * 3 + [vp]
+    L.p3dfft_poisson_multiply.restype = ctypes.c_int
     L.p3dfft_amd_prepare_length.argtypes = [ctypes.c_int]
     L.p3dfft_amd_prepare_length.restype = ctypes.c_int
     L._serve_bound = True
@@ -215,6 +217,35 @@ class PoissonServer:
                 time.sleep(2e-5)
         makespan = last_done_ms * 1e-3 - float(arrivals[0])
         return lat, makespan, max_used
+
+    def run_api(self, order):
+        """One request at a time through the reference's entry points, as a
+        caller of its single global plan does: p3dfft_setup for the request's
+        grid, ftran_r2c 'fft' into a spectrum buffer, the multiply,
+        btran_c2r 'tff' back into the field, p3dfft_clean, synchronise.
+        Returns the makespan (s)."""
+        from . import P3DFFT
+        L = self.lib
+        P = P3DFFT(precision=self.precision)
+        spec_bytes = max((a // 2 + 1) * b * c for (a, b, c) in self.shapes) * 2 * self.rsize
+        spec = L.p3dfft_amd_malloc(spec_bytes)
+        if not spec:
+            raise P3DFFTError("spectrum allocation failed")
+        try:
+            L.p3dfft_amd_synchronize()
+            t0 = time.perf_counter()
+            for i in order:
+                nx, ny, nz = self.shapes[i]
+                P.setup((1, 1), nx, ny, nz)
+                P.ftran_r2c_dev(self.field_ptr(i), spec, "fft")
+                if L.p3dfft_poisson_multiply(spec, nx, ny, nz, None):
+                    raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+                P.btran_c2r_dev(spec, self.field_ptr(i), "tff")
+                P.clean()
+                L.p3dfft_amd_synchronize()
+            return time.perf_counter() - t0
+        finally:
+            L.p3dfft_amd_free(spec)
 
     def close(self):
         L = self.lib

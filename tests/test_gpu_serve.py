@@ -127,3 +127,19 @@ def test_fused_zsolve_matches_composed(precision):
             L.p3dfft_poisson_set_fused(1)
     finally:
         S.close()
+
+
+def test_reference_api_one_at_a_time_matches():
+    """The one-at-a-time baseline through the reference's entry points (setup,
+    ftran_r2c, multiply, btran_c2r, clean per request) solves the same
+    problems: FP64 within 1e-12 of the numpy FP64 solve."""
+    shapes = serve.make_requests(6, seed=41, sizes=(32, 48, 64))
+    S = serve.PoissonServer(shapes, precision=8, slots=1)
+    try:
+        S.fill()
+        rhs = [S.download(i) for i in range(len(shapes))]
+        assert S.run_api(list(range(len(shapes)))) > 0
+        for i in range(len(shapes)):
+            assert rel(S.download(i), fp64_solve(rhs[i])) < 1e-12, shapes[i]
+    finally:
+        S.close()
