@@ -43,6 +43,12 @@ void p3dfft_get_dims(int *start, int *end, int *size, int *conf);
  * third selects the Z transform: 't'/'f' FFT, 'c' DCT-I, 's' DST-I, 'n'/'0' none. */
 void p3dfft_ftran_r2c(p3dfft_real *A, p3dfft_real *B, unsigned char *op);
 
+/* build/ftran.F90:787-814 (module procedure p3dfft_ftran_r2c_1d): the X r2c
+ * stage alone, real (nx, jisize, kjsize) -> complex (nx/2+1, jisize, kjsize),
+ * all modes, unnormalised.  Host arrays. */
+void p3dfft_ftran_r2c_1d(p3dfft_real *A, p3dfft_real *B);
+/* Extension: the same on HBM arrays, asynchronous on `stream`. */
+int p3dfft_ftran_r2c_1d_dev(const p3dfft_real *in, p3dfft_real *out, void *stream);
 /* build/btran.F90:377 (p3dfft_btran_c2r_w).  op(1:1) selects the Z transform. */
 void p3dfft_btran_c2r(p3dfft_real *A, p3dfft_real *B, unsigned char *op);
 
@@ -178,6 +184,10 @@ void p3dfft_amd_set_timing(int on);
 /* Compile-time specialised kernels for the lengths that have them (default 1);
  * 0 sends every length through the run-time-generic kernels. */
 void p3dfft_amd_set_specialised(int on);
+/* 1 (default): consecutive passes over a buffer walk their tiles in opposite
+ * orders, so each pass starts on data the previous one wrote last (still in
+ * the last-level cache); 0: every pass walks forward. Results are identical. */
+void p3dfft_amd_set_tile_order(int alternate);
 /* sizeof(p3dfft_real) of this build: 8 or 4. */
 int p3dfft_amd_precision(void);
 

@@ -64,6 +64,10 @@ def load(precision=8):
         getattr(L, n).restype = ctypes.c_int
     L.p3dfft_cheby_dev.argtypes = [_vp, _vp, rt, _vp]
     L.p3dfft_cheby_dev.restype = ctypes.c_int
+    L.p3dfft_ftran_r2c_1d.argtypes = [_vp, _vp]
+    L.p3dfft_ftran_r2c_1d.restype = None
+    L.p3dfft_ftran_r2c_1d_dev.argtypes = [_vp, _vp, _vp]
+    L.p3dfft_ftran_r2c_1d_dev.restype = ctypes.c_int
     for n in ("p3dfft_ftran_r2c_many", "p3dfft_btran_c2r_many"):
         getattr(L, n).argtypes = [_vp, _ip, _vp, _ip, _ip, ctypes.c_char_p]
         getattr(L, n).restype = None
@@ -81,6 +85,7 @@ def load(precision=8):
     L.p3dfft_amd_set_timing.argtypes = [ctypes.c_int]
     L.p3dfft_amd_precision.restype = ctypes.c_int
     L.p3dfft_amd_set_specialised.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_set_tile_order.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_chunk_bytes.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_set_chunk_bytes.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_malloc.argtypes = [ctypes.c_size_t]
@@ -209,6 +214,21 @@ class P3DFFT:
         self.lib.p3dfft_ftran_r2c(a.ctypes.data, out.ctypes.data, _op(op))
         self._raise_if_error()
         return out
+
+    def ftran_r2c_1d(self, a):
+        """p3dfft_ftran_r2c_1d (build/ftran.F90:787-814): X r2c only, all
+        nx/2+1 modes, (nz, ny, nx) real -> (nz, ny, nx/2+1) complex."""
+        a = np.ascontiguousarray(a, dtype=self.rdtype).reshape(self.real_shape())
+        nz, ny, nx = self.real_shape()
+        out = np.empty((nz, ny, nx // 2 + 1), dtype=self.cdtype)
+        self.lib.p3dfft_ftran_r2c_1d(a.ctypes.data, out.ctypes.data)
+        self._raise_if_error()
+        return out
+
+    def ftran_r2c_1d_dev(self, in_ptr, out_ptr, stream=None):
+        if self.lib.p3dfft_ftran_r2c_1d_dev(in_ptr, out_ptr, stream):
+            self._raise_if_error()
+            raise P3DFFTError("p3dfft_ftran_r2c_1d_dev failed")
 
     def btran_c2r(self, spec, op="tff"):
         s = np.ascontiguousarray(spec, dtype=self.cdtype).reshape(self.spec_shape())

@@ -228,6 +228,7 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
     y.n_inner = nxhpc; y.n_outer = nzk; y.n = ny; y.kind = COL_C2C; y.fwd = true;
     y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
     y.nst = nyc; y.st_lo = S.nycph; y.st_shift = dny;
+    y.rev = tile_order_alternate();  // starts where the X pass finished
     tstart(S, 7, s);
     launch_col(y, s);
     tstop(S, 7, s);
@@ -279,6 +280,7 @@ bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hi
       z.n = nz; z.kind = kind; z.fwd = false;
       z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
       z.nst = nz; z.st_lo = nz; z.st_shift = 0;
+      z.rev = tile_order_alternate();  // starts where a forward Z pass finished
       launch_col(z, s);
     }
     zdst = in;
@@ -320,7 +322,8 @@ bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hi
     launch_col(y, s);
     tstop(S, 10, s);
     tstart(S, 12, s);
-    launch_x_c2r(S.w1, out + (size_t)z0 * ny * nx, (long long)ny * nzk, nxhpc, nx, nx, nxhpc, s);
+    launch_x_c2r(S.w1, out + (size_t)z0 * ny * nx, (long long)ny * nzk, nxhpc, nx, nx, nxhpc, s,
+                 tile_order_alternate());  // starts where the Y pass finished
     tstop(S, 12, s);
   }
   return true;
@@ -653,6 +656,34 @@ void p3dfft_ftran_r2c(real_t* A, real_t* B, unsigned char* op) {
   GUARD_END
 }
 
+// build/ftran.F90:787-814: the X r2c stage alone, real (nx, jisize, kjsize)
+// to complex (nxhp, jisize, kjsize) (all nx/2+1 modes, no cut), unnormalised
+int p3dfft_ftran_r2c_1d_dev(const real_t* in, real_t* out, void* stream) {
+  if (!require_full()) return -1;
+  GUARD_BEGIN
+  launch_x_r2c(in, (cplx_t*)out, (long long)g.jisize * g.kjsize, g.nx, g.nxhp, g.nx, g.nxhp, (hipStream_t)stream);
+  return 0;
+  GUARD_END
+  return -1;
+}
+
+void p3dfft_ftran_r2c_1d(real_t* A, real_t* B) {
+  if (!require_full()) return;
+  GUARD_BEGIN
+  ensure_staging();
+  const size_t nout = (size_t)g.nxhp * g.jisize * g.kjsize;
+  struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+  } d_out;
+  check_hip(hipMalloc(&d_out.p, nout * sizeof(cplx_t)), "hipMalloc(ftran_r2c_1d)");
+  check_hip(hipMemcpy(g.stage_a, A, real_elems() * sizeof(real_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  launch_x_r2c((const real_t*)g.stage_a, (cplx_t*)d_out.p, (long long)g.jisize * g.kjsize, g.nx, g.nxhp, g.nx,
+               g.nxhp, g.stream);
+  check_hip(hipMemcpy(B, d_out.p, nout * sizeof(cplx_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  GUARD_END
+}
+
 // build/btran.F90:377-392
 void p3dfft_btran_c2r(real_t* A, real_t* B, unsigned char* op) {
   if (!require_full()) return;
@@ -838,6 +869,7 @@ int p3dfft_amd_precision(void) { return (int)sizeof(real_t); }
 // Use the compile-time specialised kernels for the lengths that have them
 // (default 1); 0 runs every length through the run-time-generic kernels.
 void p3dfft_amd_set_specialised(int on) { set_specialised(on != 0); }
+void p3dfft_amd_set_tile_order(int alternate) { set_tile_order(alternate != 0); }
 
 // Device memory helpers for callers without their own HIP runtime bindings.
 void* p3dfft_amd_malloc(size_t bytes) {

@@ -132,11 +132,12 @@ int p3dfft_poisson_solve(int nx, int ny, int nz, p3amd::real_t* field, void* wor
       y.n_inner = S.nxhpc; y.n_outer = nz; y.n = ny; y.kind = COL_C2C; y.fwd = true;
       y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
       y.nst = ny; y.st_lo = ny; y.st_shift = 0;
+      y.rev = tile_order_alternate();
       launch_col(y, st);
       if (launch_zsolve(spec, S.nxhpc, ny, nz, scale, st)) {
-        y.in = spec; y.out = S.w1; y.fwd = false;
+        y.in = spec; y.out = S.w1; y.fwd = false; y.rev = false;
         launch_col(y, st);
-        launch_x_c2r(S.w1, field, (long long)ny * nz, S.nxhpc, nx, nx, S.nxhpc, st);
+        launch_x_c2r(S.w1, field, (long long)ny * nz, S.nxhpc, nx, nx, S.nxhpc, st, tile_order_alternate());
         S.w1 = nullptr;
         return 0;
       }

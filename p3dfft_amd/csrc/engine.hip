@@ -43,6 +43,12 @@ __device__ __forceinline__ int xcd_tile(int bid, int nwg) {
   return x * q + (x < rm ? x : rm) + slot;
 }
 
+// tile of this workgroup: XCD-aware, optionally walking from the last tile
+__device__ __forceinline__ int tile_of(int rev) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  return rev ? (int)gridDim.x - 1 - t : t;
+}
+
 // ---------------------------------------------------------------------------
 // Stockham stages over an LDS tile.  COL layout: element e of sequence b at
 // lds[e*B + b] (B sequences side by side, sequence index fastest).  ROW layout:
@@ -188,6 +194,7 @@ struct XArgs {
   const void* tw;
   int tw_lds;  // twiddles staged in LDS
   int tw_off;  // byte offset of the staged twiddles
+  int rev;     // walk the tiles from the last one
 };
 
 extern __shared__ __align__(16) unsigned char g_smem[];
@@ -199,7 +206,7 @@ __global__ __launch_bounds__(NT) void k_x_r2c(XArgs a) {
   const int n = a.L.n, B = a.B;
   const TwL* tw = (const TwL*)a.tw;
   if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
-  const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
+  const long long pair0 = (long long)tile_of(a.rev) * B;
   const T* in = (const T*)a.in;
   for (int q = threadIdx.x; q < B * n; q += NT) {
     const int b = fdiv(q, a.L.dn), j = q - b * n;
@@ -228,7 +235,7 @@ __global__ __launch_bounds__(NT) void k_x_c2r(XArgs a) {
   const int n = a.L.n, B = a.B, h = n / 2;
   const TwL* tw = (const TwL*)a.tw;
   if (a.tw_lds) tw = stage_twiddles<T, NT>(g_smem, a.tw_off, tw, a.L.twcount);
-  const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
+  const long long pair0 = (long long)tile_of(a.rev) * B;
   const cpx<T>* in = (const cpx<T>*)a.in;
   for (int q = threadIdx.x; q < B * n; q += NT) {
     const int b = fdiv(q, a.L.dn), k = q - b * n;
@@ -274,6 +281,7 @@ struct ColArgs {
   LenDesc L;
   const void* tw;
   int tw_lds, tw_off;
+  int rev;  // walk the tiles from the last one
 };
 
 template <class T, bool FWD, int NT, int EPT>
@@ -356,7 +364,7 @@ __global__ __launch_bounds__(NT, W) void k_x_r2c_ct(XArgs a) {
   constexpr int BP = B + 1;
   cpx<T>* lds = (cpx<T>*)g_smem;
   const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
-  const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
+  const long long pair0 = (long long)tile_of(a.rev) * B;
   // rows of this tile (2B unless it is the last one)
   const int nr = (int)(a.nrows - 2 * pair0 < 2 * B ? a.nrows - 2 * pair0 : 2 * B);
   const char* in = (const char*)a.in + 2 * pair0 * a.si_row * (long long)sizeof(T);
@@ -413,7 +421,7 @@ __global__ __launch_bounds__(NT, W) void k_x_c2r_ct(XArgs a) {
   constexpr int H = N / 2;
   cpx<T>* lds = (cpx<T>*)g_smem;
   const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
-  const long long pair0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * B;
+  const long long pair0 = (long long)tile_of(a.rev) * B;
   const int nr = (int)(a.nrows - 2 * pair0 < 2 * B ? a.nrows - 2 * pair0 : 2 * B);
   const char* in = (const char*)a.in + 2 * pair0 * a.si_row * (long long)sizeof(cpx<T>);
   const unsigned si = (unsigned)(a.si_row * sizeof(cpx<T>));
@@ -503,7 +511,7 @@ __global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
   // the twiddle staging is completed by the barrier at the end of the first stage
   const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
   const long long ncol = (long long)a.n_inner * a.n_outer;
-  const long long c0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * TX;
+  const long long c0 = (long long)tile_of(a.rev) * TX;
   const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
   const ColIO io = col_io<T, TX>(a, c0, nb);
   run_fft_io<T, FWD, TX, NT, PR>(lds, tw, io, RL{});
@@ -516,7 +524,7 @@ template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL>
 __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
   cpx<T>* lds = (cpx<T>*)g_smem;
   const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int t = tile_of(a.rev);
   const int o = t / a.tiles;
   const int ti = t - o * a.tiles;
   const long long i0 = (long long)ti * TX;
@@ -575,7 +583,7 @@ __global__ __launch_bounds__(NT, W) void k_zsolve_ct(ColArgs a, double scale, in
   cpx<T>* lds = (cpx<T>*)g_smem;
   const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
   const long long ncol = a.n_inner;
-  const long long c0 = (long long)xcd_tile(blockIdx.x, gridDim.x) * TX;
+  const long long c0 = (long long)tile_of(a.rev) * TX;
   const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
   const ColIO io = col_io<T, TX>(a, c0, nb);
   SolveCtx sc;
@@ -966,6 +974,9 @@ static const CtInfo* find_ct(const Len1D& L) {
 }
 
 static bool g_use_ct = true;
+static bool g_alt_order = true;
+void set_tile_order(bool alternate) { g_alt_order = alternate; }
+bool tile_order_alternate() { return g_alt_order; }
 void set_specialised(bool on) { g_use_ct = on; }
 
 // row-pair count per X tile
@@ -1019,13 +1030,14 @@ static bool x_setup_ct(XArgs& a, const Len1D& L, const CtInfo& c, long long nrow
 }
 
 void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_row, long long so_row, int nx,
-                  int nxo, hipStream_t st) {
+                  int nxo, hipStream_t st, bool rev) {
   if (nrows <= 0 || nxo <= 0) return;
   const Len1D& L = get_len(nx);
   XArgs a;
   std::memset(&a, 0, sizeof(a));
   a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;
   a.nxo = nxo; a.dnxo = make_fdiv(nxo);
+  a.rev = rev ? 1 : 0;
   if (const CtInfo* c = g_use_ct ? find_ct(L) : nullptr) {
     size_t bytes; unsigned nwg;
     if (x_setup_ct(a, L, *c, nrows, si_row, so_row, &bytes, &nwg)) {
@@ -1041,13 +1053,14 @@ void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_r
 }
 
 void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_row, long long so_row, int nx,
-                  int nxi, hipStream_t st) {
+                  int nxi, hipStream_t st, bool rev) {
   if (nrows <= 0) return;
   const Len1D& L = get_len(nx);
   XArgs a;
   std::memset(&a, 0, sizeof(a));
   a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;
   a.nxo = nxi; a.dnxo = make_fdiv(nxi > 0 ? nxi : 1);
+  a.rev = rev ? 1 : 0;
   if (const CtInfo* c = g_use_ct ? find_ct(L) : nullptr) {
     size_t bytes; unsigned nwg;
     if (x_setup_ct(a, L, *c, nrows, si_row, so_row, &bytes, &nwg)) {
@@ -1082,6 +1095,7 @@ void launch_col(const ColSpec& c, hipStream_t st) {
   a.si_out = c.si_out; a.sl_out = c.sl_out; a.so_out = c.so_out;
   a.n_inner = c.n_inner; a.n_outer = c.n_outer;
   a.kind = c.kind;
+  a.rev = c.rev ? 1 : 0;
   a.nld = c.nld; a.ld_lo = c.ld_lo; a.ld_hi = c.ld_hi; a.ld_shift = c.ld_shift;
   a.nst = c.nst; a.st_lo = c.st_lo; a.st_shift = c.st_shift;
   int nfft;

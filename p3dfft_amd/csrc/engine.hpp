@@ -49,9 +49,9 @@ int max_len();
 //   c2r: in complex rows with nxi valid modes (the rest read as zero),
 //        out real rows.
 void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_row, long long so_row,
-                  int nx, int nxo, hipStream_t st);
+                  int nx, int nxo, hipStream_t st, bool rev = false);
 void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_row, long long so_row,
-                  int nx, int nxi, hipStream_t st);
+                  int nx, int nxi, hipStream_t st, bool rev = false);
 
 // Column transform along a strided axis of a 3D view [outer][len][inner].
 enum ColKind { COL_C2C = 0, COL_DCT1 = 1, COL_DST1 = 2, COL_COPY = 3 };
@@ -68,8 +68,15 @@ struct ColSpec {
   int ld_lo, ld_hi, ld_shift;  // LDS row e <- input row e (e<ld_lo), e-ld_shift (e>=ld_hi), else 0
   int nst;        // rows written to the output
   int st_lo, st_shift;         // output row e <- LDS row e (e<st_lo) or e+st_shift
+  bool rev;       // walk the tiles from the last one (see set_tile_order)
 };
 void launch_col(const ColSpec& c, hipStream_t st);
+
+// Consecutive passes over one buffer walk their tiles in opposite orders, so
+// a pass starts on what the previous pass wrote last (still in the
+// last-level cache).  Off: every pass walks forward.
+void set_tile_order(bool alternate);
+bool tile_order_alternate();
 
 // Fused Z-direction spectral solve on a spectrum (nxh, ny, nz) in place:
 // forward FFT along z, multiply by -scale/|k|^2 (k=0 -> 0), inverse FFT.

@@ -20,10 +20,15 @@ module p3dfft
   integer, parameter, public :: i8 = selected_int_kind(18)
   ! stage timers (build/module.F90:106), refreshed after every transform call
   real(r8), save, public :: timers(12) = 0.0_r8
+  ! MPI datatypes of p3dfft_type reals and complexes (build/module.F90:95-104).
+  ! This library does not link MPI: they hold -1 until a caller that uses MPI
+  ! sets them with p3dfft_set_mpi_types (e.g. MPI_DOUBLE_PRECISION and
+  ! MPI_DOUBLE_COMPLEX), after which they can be used as in the reference.
+  integer, save, public :: p3dfft_mpireal = -1, p3dfft_mpicomplex = -1
 
   public :: p3dfft_setup, p3dfft_get_dims, p3dfft_ftran_r2c, p3dfft_btran_c2r, &
             p3dfft_cheby, p3dfft_ftran_r2c_many, p3dfft_btran_c2r_many, p3dfft_cheby_many, &
-            p3dfft_clean, get_timers, set_timers
+            p3dfft_ftran_r2c_1d, p3dfft_clean, get_timers, set_timers, p3dfft_set_mpi_types
 
   interface
     subroutine c_p3dfft_setup(dims, nx, ny, nz, comm, nxc, nyc, nzc, ow, memsize) &
@@ -65,6 +70,10 @@ module p3dfft
       import :: c_ptr, c_int
       type(c_ptr), value :: a, b, lz
       integer(c_int) :: dim_in, dim_out, nv
+    end subroutine
+    subroutine c_p3dfft_ftran_r2c_1d(a, b) bind(C, name='p3dfft_ftran_r2c_1d')
+      import :: c_ptr
+      type(c_ptr), value :: a, b
     end subroutine
     subroutine c_p3dfft_clean() bind(C, name='p3dfft_clean')
     end subroutine
@@ -174,6 +183,18 @@ contains
     di = dim_in; do_ = dim_out; n = nv; lzc = Lz
     call c_p3dfft_cheby_many(c_loc(in(1)), di, c_loc(out(1)), do_, n, c_loc(lzc))
     call c_get_timers(timers)
+  end subroutine
+
+  ! build/ftran.F90:787-814: X r2c only; cXgYZ holds (nx/2+1) complex per row
+  subroutine p3dfft_ftran_r2c_1d(rXgYZ, cXgYZ)
+    real(p3dfft_type), target :: rXgYZ(*), cXgYZ(*)
+    call c_p3dfft_ftran_r2c_1d(c_loc(rXgYZ(1)), c_loc(cXgYZ(1)))
+  end subroutine
+
+  subroutine p3dfft_set_mpi_types(real_type, complex_type)
+    integer, intent(in) :: real_type, complex_type
+    p3dfft_mpireal = real_type
+    p3dfft_mpicomplex = complex_type
   end subroutine
 
   ! build/module.F90:309
