@@ -188,6 +188,9 @@ void p3dfft_amd_set_specialised(int on);
  * orders, so each pass starts on data the previous one wrote last (still in
  * the last-level cache); 0: every pass walks forward. Results are identical. */
 void p3dfft_amd_set_tile_order(int alternate);
+/* 1: the specialised column kernels use 64-bit element offsets even where
+ * 32-bit ones fit (default 0; views over 4 GiB always use 64-bit offsets). */
+void p3dfft_amd_set_wide_offsets(int on);
 /* sizeof(p3dfft_real) of this build: 8 or 4. */
 int p3dfft_amd_precision(void);
 

@@ -870,6 +870,7 @@ int p3dfft_amd_precision(void) { return (int)sizeof(real_t); }
 // (default 1); 0 runs every length through the run-time-generic kernels.
 void p3dfft_amd_set_specialised(int on) { set_specialised(on != 0); }
 void p3dfft_amd_set_tile_order(int alternate) { set_tile_order(alternate != 0); }
+void p3dfft_amd_set_wide_offsets(int on) { set_wide_offsets(on != 0); }
 
 // Device memory helpers for callers without their own HIP runtime bindings.
 void* p3dfft_amd_malloc(size_t bytes) {

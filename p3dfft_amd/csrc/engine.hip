@@ -492,10 +492,10 @@ __device__ __forceinline__ ColIO col_io(const ColArgs& a, long long c0, int nb) 
   constexpr long long E = sizeof(cpx<T>);
   io.in = (const char*)a.in + ((long long)o0 * a.so_in + (long long)i0 * a.si_in) * E;
   io.out = (char*)a.out + ((long long)o0 * a.so_out + (long long)i0 * a.si_out) * E;
-  io.lane_in = (unsigned)((((long long)o - o0) * a.so_in + ((long long)i - i0) * a.si_in) * E);
-  io.lane_out = (unsigned)((((long long)o - o0) * a.so_out + ((long long)i - i0) * a.si_out) * E);
-  io.sl_in = (unsigned)(a.sl_in * E);
-  io.sl_out = (unsigned)(a.sl_out * E);
+  io.lane_in = (unsigned long long)((((long long)o - o0) * a.so_in + ((long long)i - i0) * a.si_in) * E);
+  io.lane_out = (unsigned long long)((((long long)o - o0) * a.so_out + ((long long)i - i0) * a.si_out) * E);
+  io.sl_in = (unsigned long long)(a.sl_in * E);
+  io.sl_out = (unsigned long long)(a.sl_out * E);
   io.nb = nb;
   io.ld_lo = a.ld_lo; io.ld_hi = a.ld_hi; io.ld_shift = a.ld_shift;
   io.nst = a.nst; io.st_lo = a.st_lo; io.st_shift = a.st_shift;
@@ -505,7 +505,7 @@ __device__ __forceinline__ ColIO col_io(const ColArgs& a, long long c0, int nb) 
 // C2C column transform with direct global <-> register first/last stages,
 // over tiles of TX consecutive columns of the flattened (outer, inner) index
 // (no tile is partial except the last one).  PR: pruning maps.
-template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL, bool PR>
+template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL, bool PR, class Off>
 __global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
   cpx<T>* lds = (cpx<T>*)g_smem;
   // the twiddle staging is completed by the barrier at the end of the first stage
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
   const long long c0 = (long long)tile_of(a.rev) * TX;
   const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
   const ColIO io = col_io<T, TX>(a, c0, nb);
-  run_fft_io<T, FWD, TX, NT, PR>(lds, tw, io, RL{});
+  run_fft_io<T, FWD, TX, NT, PR, Off>(lds, tw, io, RL{});
 }
 
 // Column transforms through an LDS load/store pass: DCT-I / DST-I
@@ -883,7 +883,7 @@ struct CtInfo {
   bool twl_col, twl_x;  // twiddles staged in LDS (column / X kernels)
   long long twb;        // twiddle table bytes
   void (*col)(bool, const ColArgs&, unsigned, size_t, hipStream_t);
-  void (*colio)(bool, bool, const ColArgs&, unsigned, size_t, hipStream_t);
+  void (*colio)(bool, bool, bool, const ColArgs&, unsigned, size_t, hipStream_t);
   void (*xr2c)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*xc2r)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*zsolve)(const ColArgs&, double, int, int, unsigned, size_t, hipStream_t);
@@ -921,13 +921,18 @@ struct CtLen {
     if (fwd) go(k_col_ct<real_t, true, TX, NTC, RL, WC, TWLC>, g, NTC, bytes, s, a);
     else go(k_col_ct<real_t, false, TX, NTC, RL, WC, TWLC>, g, NTC, bytes, s, a);
   }
-  static void colio(bool fwd, bool pr, const ColArgs& a, unsigned g, size_t bytes, hipStream_t s) {
+  // pr: pruning maps; wide: 64-bit offsets (unpruned views only)
+  static void colio(bool fwd, bool pr, bool wide, const ColArgs& a, unsigned g, size_t bytes, hipStream_t s) {
+    typedef unsigned U32;
+    typedef unsigned long long U64;
     if (fwd) {
-      if (pr) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, true>, g, NTC, bytes, s, a);
-      else go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, false>, g, NTC, bytes, s, a);
+      if (pr) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, true, U32>, g, NTC, bytes, s, a);
+      else if (wide) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, false, U64>, g, NTC, bytes, s, a);
+      else go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, false, U32>, g, NTC, bytes, s, a);
     } else {
-      if (pr) go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, true>, g, NTC, bytes, s, a);
-      else go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, false>, g, NTC, bytes, s, a);
+      if (pr) go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, true, U32>, g, NTC, bytes, s, a);
+      else if (wide) go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, false, U64>, g, NTC, bytes, s, a);
+      else go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, false, U32>, g, NTC, bytes, s, a);
     }
   }
   static void xr2c(const XArgs& a, unsigned g, size_t bytes, hipStream_t s) {
@@ -975,6 +980,8 @@ static const CtInfo* find_ct(const Len1D& L) {
 
 static bool g_use_ct = true;
 static bool g_alt_order = true;
+static bool g_wide_offsets = false;
+void set_wide_offsets(bool on) { g_wide_offsets = on; }
 void set_tile_order(bool alternate) { g_alt_order = alternate; }
 bool tile_order_alternate() { return g_alt_order; }
 void set_specialised(bool on) { g_use_ct = on; }
@@ -1130,12 +1137,13 @@ void launch_col(const ColSpec& c, hipStream_t st) {
     if (bytes > kLdsMax) throw Error{"column transform does not fit in LDS", -1};
     const long long ncol = (long long)c.n_inner * c.n_outer;
     if (ncol > 0x7fffffffLL) throw Error{"grid too large", -1};
-    if (c.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast && col_fits32(c, TX, rows)) {
-      // identity row maps (no zero gap, no shift, all rows): no pruning
-      const bool ld_id = c.nld == nfft && (c.ld_lo >= nfft || (c.ld_hi <= c.ld_lo && c.ld_shift == 0));
-      const bool st_id = c.nst == nfft && (c.st_lo >= nfft || c.st_shift == 0);
-      const bool pr = !(ld_id && st_id);
-      ct->colio(fwd, pr, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
+    // identity row maps (no zero gap, no shift, all rows): no pruning
+    const bool ld_id = c.nld == nfft && (c.ld_lo >= nfft || (c.ld_hi <= c.ld_lo && c.ld_shift == 0));
+    const bool st_id = c.nst == nfft && (c.st_lo >= nfft || c.st_shift == 0);
+    const bool pr = !(ld_id && st_id);
+    const bool fits = col_fits32(c, TX, rows) && !g_wide_offsets;
+    if (c.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast && (fits || !pr)) {
+      ct->colio(fwd, pr, !fits, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
       check_hip(hipGetLastError(), "k_colio_ct launch");
       return;
     }

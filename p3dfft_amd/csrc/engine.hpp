@@ -76,6 +76,9 @@ void launch_col(const ColSpec& c, hipStream_t st);
 // a pass starts on what the previous pass wrote last (still in the
 // last-level cache).  Off: every pass walks forward.
 void set_tile_order(bool alternate);
+// Specialised column kernels address with 64-bit offsets even where 32-bit
+// ones fit (views larger than 4 GiB always use 64-bit ones).
+void set_wide_offsets(bool on);
 bool tile_order_alternate();
 
 // Fused Z-direction spectral solve on a spectrum (nxh, ny, nz) in place:

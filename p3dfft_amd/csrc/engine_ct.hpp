@@ -82,15 +82,17 @@ __device__ __forceinline__ void run_fft_ct(cpx<T>* lds, const typename TwT<T>::t
 // the last stage stored straight to global memory (one LDS round trip fewer
 // at each end; every thread keeps IT*R independent loads in flight).
 // Addresses are a wave-uniform 64-bit tile base plus 32-bit per-lane byte
-// offsets, so an element access costs one 32-bit add.  With PR the rows are
+// offsets (Off = unsigned), so an element access costs one 32-bit add; views
+// whose offsets do not fit use 64-bit offsets (Off = unsigned long long).
+// With PR the rows are
 // remapped on load (pruning zero-fill) and on store (pruning selection);
 // without it both maps are the identity.
 // ---------------------------------------------------------------------------
 struct ColIO {
   const char* in;             // input tile base
   char* out;                  // output tile base
-  unsigned lane_in, lane_out; // this lane's column, bytes from the tile bases
-  unsigned sl_in, sl_out;     // row strides, bytes
+  unsigned long long lane_in, lane_out;  // this lane's column, bytes from the tile bases
+  unsigned long long sl_in, sl_out;      // row strides, bytes
   int nb;                     // valid columns of this tile
   int ld_lo, ld_hi, ld_shift; // PR: LDS row e <- input row e (e<ld_lo), e-ld_shift (e>=ld_hi), else 0
   int nst, st_lo, st_shift;   // PR: output row e <- FFT row e (e<st_lo) or e+st_shift
@@ -103,19 +105,19 @@ struct Stage0 {
   static constexpr int IT = (total + NT - 1) / NT;
 };
 
-template <class T>
-__device__ __forceinline__ cpx<T> ld_at(const char* base, unsigned off) {
+template <class T, class Off>
+__device__ __forceinline__ cpx<T> ld_at(const char* base, Off off) {
   return *(const cpx<T>*)(base + off);
 }
 
-template <class T>
-__device__ __forceinline__ void st_at(char* base, unsigned off, cpx<T> v) {
+template <class T, class Off>
+__device__ __forceinline__ void st_at(char* base, Off off, cpx<T> v) {
   *(cpx<T>*)(base + off) = v;
 }
 
 // first stage: loads of one tile into registers, butterflies (no twiddles),
 // results into the LDS tile
-template <class T, int R, bool FWD, int N, int B, int NT, bool PR>
+template <class T, int R, bool FWD, int N, int B, int NT, bool PR, class Off>
 __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
   static_assert(NT % B == 0, "one column per thread");
   typedef Stage0<R, N, B, NT> S;
@@ -134,10 +136,10 @@ __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
         if constexpr (PR) {
           const int src = e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1);
           const bool ok = valid && src >= 0;
-          const cpx<T> u = ld_at<T>(io.in, ok ? io.lane_in + (unsigned)src * io.sl_in : 0u);
+          const cpx<T> u = ld_at<T, Off>(io.in, ok ? (Off)io.lane_in + (Off)src * (Off)io.sl_in : (Off)0);
           v[m][r] = ok ? u : mk<T>(0, 0);
         } else {
-          const cpx<T> u = ld_at<T>(io.in, valid ? io.lane_in + (unsigned)e * io.sl_in : 0u);
+          const cpx<T> u = ld_at<T, Off>(io.in, valid ? (Off)io.lane_in + (Off)e * (Off)io.sl_in : (Off)0);
           v[m][r] = valid ? u : mk<T>(0, 0);
         }
       }
@@ -157,7 +159,7 @@ __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
 }
 
 // last stage: Ns * R == N, so every butterfly index j < Ns and k == j
-template <class T, int R, bool FWD, int N, int B, int NT, int Ns, int Off, bool PR>
+template <class T, int R, bool FWD, int N, int B, int NT, int Ns, int TwOff, bool PR, class Off>
 __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename TwT<T>::type* tw,
                                               const ColIO& io) {
   static_assert(Ns * R == N, "last stage");
@@ -173,7 +175,7 @@ __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename 
       cpx<T> v[R];
 #pragma unroll
       for (int r = 0; r < R; r++) v[r] = lds[(j + r * Ns) * B + b];
-      const typename TwT<T>::type* w = tw + Off + j * (R - 1);
+      const typename TwT<T>::type* w = tw + TwOff + j * (R - 1);
 #pragma unroll
       for (int r = 1; r < R; r++) v[r] = twmul<FWD>(v[r], w[r - 1]);
       dft<R, FWD>(v);
@@ -184,9 +186,9 @@ __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename 
           if constexpr (PR) {
             const int e = p < io.st_lo ? p : p - io.st_shift;
             if (p < io.st_lo || (p >= io.st_lo + io.st_shift && e < io.nst))
-              st_at<T>(io.out, io.lane_out + (unsigned)e * io.sl_out, v[r]);
+              st_at<T, Off>(io.out, (Off)io.lane_out + (Off)e * (Off)io.sl_out, v[r]);
           } else {
-            st_at<T>(io.out, io.lane_out + (unsigned)p * io.sl_out, v[r]);
+            st_at<T, Off>(io.out, (Off)io.lane_out + (Off)p * (Off)io.sl_out, v[r]);
           }
         }
       }
@@ -194,23 +196,23 @@ __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename 
   }
 }
 
-template <class T, bool FWD, int N, int B, int NT, int Ns, int Off, bool PR, int R, int... Rest>
+template <class T, bool FWD, int N, int B, int NT, int Ns, int TwOff, bool PR, class Off, int R, int... Rest>
 __device__ __forceinline__ void mid_and_last(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO& io) {
   if constexpr (sizeof...(Rest) == 0) {
-    stageL_global<T, R, FWD, N, B, NT, Ns, Off, PR>(lds, tw, io);
+    stageL_global<T, R, FWD, N, B, NT, Ns, TwOff, PR, Off>(lds, tw, io);
   } else {
-    stage_ct<T, R, FWD, N, B, B, NT, Ns, Off>(lds, tw);
-    mid_and_last<T, FWD, N, B, NT, Ns * R, (Ns > 1 ? Off + Ns * (R - 1) : Off), PR, Rest...>(lds, tw, io);
+    stage_ct<T, R, FWD, N, B, B, NT, Ns, TwOff>(lds, tw);
+    mid_and_last<T, FWD, N, B, NT, Ns * R, (Ns > 1 ? TwOff + Ns * (R - 1) : TwOff), PR, Off, Rest...>(lds, tw, io);
   }
 }
 
-template <class T, bool FWD, int B, int NT, bool PR, int R0, int... Rs>
+template <class T, bool FWD, int B, int NT, bool PR, class Off, int R0, int... Rs>
 __device__ __forceinline__ void run_fft_io(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO& io,
                                            RList<R0, Rs...>) {
   static_assert(sizeof...(Rs) >= 1, "at least two stages");
   constexpr int N = R0 * RProd<Rs...>::value;
-  stage0_global<T, R0, FWD, N, B, NT, PR>(lds, io);
-  mid_and_last<T, FWD, N, B, NT, R0, 0, PR, Rs...>(lds, tw, io);
+  stage0_global<T, R0, FWD, N, B, NT, PR, Off>(lds, io);
+  mid_and_last<T, FWD, N, B, NT, R0, 0, PR, Off, Rs...>(lds, tw, io);
 }
 
 // Spectral solve along the column: forward FFT, per-mode multiply by
@@ -226,7 +228,7 @@ __device__ __forceinline__ void run_fft_solve(cpx<T>* lds, const typename TwT<T>
                                               long long i0, const SolveCtx& sc, RList<R0, Rs...>) {
   static_assert(sizeof...(Rs) >= 1, "at least two stages");
   constexpr int N = R0 * RProd<Rs...>::value;
-  stage0_global<T, R0, true, N, B, NT, false>(lds, io);
+  stage0_global<T, R0, true, N, B, NT, false, unsigned>(lds, io);
   stages_ct<T, true, N, B, B, NT, R0, 0, Rs...>(lds, tw);
   for (int q = threadIdx.x; q < B * N; q += NT) {
     const int e = q / B, b = q % B;
@@ -241,7 +243,7 @@ __device__ __forceinline__ void run_fft_solve(cpx<T>* lds, const typename TwT<T>
     lds[q] = mk<T>((T)(v.x * f), (T)(v.y * f));
   }
   __syncthreads();
-  mid_and_last<T, false, N, B, NT, 1, 0, false, R0, Rs...>(lds, tw, io);
+  mid_and_last<T, false, N, B, NT, 1, 0, false, unsigned, R0, Rs...>(lds, tw, io);
 }
 
 // Twiddle table entries used by a radix list (all stages after the first).

@@ -421,3 +421,24 @@ def test_tile_order_bit_identical(precision):
             P.clean()
             P.lib.p3dfft_amd_set_tile_order(1)
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape,ops", [((64, 48, 40), ("fft", "tff")), ((36, 20, 30), ("fft", "tff"))])
+def test_wide_offsets_bit_identical(precision, shape, ops):
+    """The 64-bit-offset column kernels (the path of views over 4 GiB) give
+    the same bits as the 32-bit-offset ones."""
+    nx, ny, nz = shape
+    f = np.random.default_rng(23).standard_normal((nz, ny, nx))
+    res = []
+    for wide in (0, 1):
+        P = p3dfft_amd.P3DFFT(precision)
+        P.lib.p3dfft_amd_set_wide_offsets(wide)
+        P.setup((1, 1), nx, ny, nz)
+        try:
+            s = P.ftran_r2c(f, ops[0])
+            res.append((s, P.btran_c2r(s, ops[1])))
+        finally:
+            P.clean()
+            P.lib.p3dfft_amd_set_wide_offsets(0)
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
