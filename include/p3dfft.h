@@ -191,6 +191,11 @@ void p3dfft_amd_set_tile_order(int alternate);
 /* 1: the specialised column kernels use 64-bit element offsets even where
  * 32-bit ones fit (default 0; views over 4 GiB always use 64-bit offsets). */
 void p3dfft_amd_set_wide_offsets(int on);
+/* 1 (default): a forward without Y/Z cuts whose output does not overlap its
+ * input runs X r2c straight into the output and the Y pass in place there
+ * (the Z pass then reads lines the Y pass left in the last-level cache);
+ * 0: X r2c into the work buffer, Y from there into the output. */
+void p3dfft_amd_set_x_into_out(int on);
 /* sizeof(p3dfft_real) of this build: 8 or 4. */
 int p3dfft_amd_precision(void);
 

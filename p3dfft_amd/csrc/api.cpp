@@ -191,6 +191,10 @@ int chunk_planes(State& S) {
   return (int)std::min<size_t>(zc, (size_t)S.nz);
 }
 
+// forward without Y/Z cuts: X r2c into the output, Y in place (see
+// p3dfft_amd_set_x_into_out)
+static bool g_x_into_out = true;
+
 // ---------------------------------------------------------------------------
 // forward: X r2c -> Y c2c (per z-plane) -> Z op, build/ftran.F90:489-780
 // ---------------------------------------------------------------------------
@@ -216,7 +220,27 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
   const bool alias = ob < ib + (size_t)nx * ny * nz * sizeof(real_t) &&
                      ib < ob + (size_t)nxhpc * nyc * nz * sizeof(cplx_t);
   const int zc = alias ? nz : chunk_planes(S);
-  for (int z0 = 0; z0 < nz; z0 += zc) {
+  // unpruned in Y, no overlap: X r2c straight into the output and Y in place
+  // there, so the Y pass reads (and leaves in the last-level cache) the lines
+  // the Z pass reads next
+  const bool xin = g_x_into_out && direct && dny == 0 && !alias && zc == nz;
+  if (xin) {
+    tstart(S, 5, s);
+    launch_x_r2c(in, out, (long long)ny * nz, nx, nxhpc, nx, nxhpc, s);
+    tstop(S, 5, s);
+    ColSpec y{};
+    y.in = out; y.out = out;
+    y.si_in = 1; y.sl_in = nxhpc; y.so_in = nxy;
+    y.si_out = 1; y.sl_out = nxhpc; y.so_out = nxy;
+    y.n_inner = nxhpc; y.n_outer = nz; y.n = ny; y.kind = COL_C2C; y.fwd = true;
+    y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
+    y.nst = ny; y.st_lo = ny; y.st_shift = 0;
+    y.rev = tile_order_alternate();
+    tstart(S, 7, s);
+    launch_col(y, s);
+    tstop(S, 7, s);
+  }
+  for (int z0 = 0; z0 < (xin ? 0 : nz); z0 += zc) {
     const int nzk = std::min(zc, nz - z0);
     tstart(S, 5, s);
     launch_x_r2c(in + (size_t)z0 * ny * nx, S.w1, (long long)ny * nzk, nx, nxhpc, nx, nxhpc, s);
@@ -871,6 +895,7 @@ int p3dfft_amd_precision(void) { return (int)sizeof(real_t); }
 void p3dfft_amd_set_specialised(int on) { set_specialised(on != 0); }
 void p3dfft_amd_set_tile_order(int alternate) { set_tile_order(alternate != 0); }
 void p3dfft_amd_set_wide_offsets(int on) { set_wide_offsets(on != 0); }
+void p3dfft_amd_set_x_into_out(int on) { g_x_into_out = (on != 0); }
 
 // Device memory helpers for callers without their own HIP runtime bindings.
 void* p3dfft_amd_malloc(size_t bytes) {

@@ -56,24 +56,13 @@ __global__ void k_fill_normal(T* p, long long n, uint64_t seed) {
   p[i] = (T)(sqrt(-2.0 * log(u1)) * cospi(2.0 * u2));
 }
 
-// geometry of a full, unpruned 1x1 transform of (nx, ny, nz) with overwrite
-static void poisson_geometry(State& S, int nx, int ny, int nz) {
-  S.nx = nx; S.ny = ny; S.nz = nz;
-  S.nxc = nx; S.nyc = ny; S.nzc = nz;
-  S.ow = true;
-  S.stride1 = false;
-  S.timing = false;
-  const int dims[2] = {1, 1};
-  compute_geometry(S, dims, 0, 0);
-}
-
 // fused Z forward / multiply / Z inverse kernel (default on); off composes
 // the solve from the forward transform, a multiply pass and the backward one
 static bool g_fused_solve = true;
 
 size_t poisson_work_bytes(int nx, int ny, int nz) {
-  // spectrum + X/Y intermediate
-  return 2 * (size_t)(nx / 2 + 1) * ny * nz * sizeof(cplx_t);
+  // the spectrum: every pass after X r2c runs in place on it
+  return (size_t)(nx / 2 + 1) * ny * nz * sizeof(cplx_t);
 }
 
 }  // namespace p3amd
@@ -113,53 +102,42 @@ int p3dfft_poisson_solve(int nx, int ny, int nz, p3amd::real_t* field, void* wor
       set_error("p3dfft_poisson_solve: work area too small");
       return -1;
     }
-    State S;
-    poisson_geometry(S, nx, ny, nz);
-    const size_t nc = (size_t)S.nxhpc * ny * nz;
+    const int nxhp = nx / 2 + 1;
+    const long long nxy = (long long)nxhp * ny;
     cplx_t* spec = (cplx_t*)work;
-    S.w1 = spec + nc;
-    S.w2 = nullptr;  // unpruned, (X,Y,Z) layout, overwrite: never touched
     hipStream_t st = (hipStream_t)stream;
     const double scale = 1.0 / ((double)nx * ny * nz);
-    if (g_fused_solve) {
-      // X r2c, Y c2c, fused Z forward / multiply / Z inverse, Y c2c, X c2r
-      const long long nxy = (long long)S.nxhpc * ny;
-      launch_x_r2c(field, S.w1, (long long)ny * nz, nx, S.nxhpc, nx, S.nxhpc, st);
-      ColSpec y{};
-      y.in = S.w1; y.out = spec;
-      y.si_in = 1; y.sl_in = S.nxhpc; y.so_in = nxy;
-      y.si_out = 1; y.sl_out = S.nxhpc; y.so_out = nxy;
-      y.n_inner = S.nxhpc; y.n_outer = nz; y.n = ny; y.kind = COL_C2C; y.fwd = true;
-      y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
-      y.nst = ny; y.st_lo = ny; y.st_shift = 0;
-      y.rev = tile_order_alternate();
-      launch_col(y, st);
-      if (launch_zsolve(spec, S.nxhpc, ny, nz, scale, st)) {
-        y.in = spec; y.out = S.w1; y.fwd = false; y.rev = false;
-        launch_col(y, st);
-        launch_x_c2r(S.w1, field, (long long)ny * nz, S.nxhpc, nx, nx, S.nxhpc, st, tile_order_alternate());
-        S.w1 = nullptr;
-        return 0;
+    const bool alt = tile_order_alternate();
+    // every pass after X r2c runs in place on the spectrum; consecutive
+    // passes walk their tiles in opposite orders (the last-level cache keeps
+    // what the previous pass read)
+    auto col = [&](int axis, bool fwd, bool rev) {
+      ColSpec c{};
+      c.in = spec; c.out = spec;
+      c.si_in = 1; c.si_out = 1;
+      if (axis == 1) {
+        c.sl_in = c.sl_out = nxhp; c.so_in = c.so_out = nxy; c.n_inner = nxhp; c.n_outer = nz; c.n = ny;
+      } else {
+        c.sl_in = c.sl_out = nxy; c.so_in = c.so_out = 0; c.n_inner = (int)nxy; c.n_outer = 1; c.n = nz;
       }
-      // no specialised Z length: Z forward here, then the composed tail below
-      ColSpec z{};
-      z.in = spec; z.out = spec;
-      z.n_inner = (int)nxy; z.n_outer = 1;
-      z.si_in = 1; z.sl_in = nxy; z.si_out = 1; z.sl_out = nxy;
-      z.n = nz; z.kind = COL_C2C; z.fwd = true;
-      z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
-      z.nst = nz; z.st_lo = nz; z.st_shift = 0;
-      launch_col(z, st);
-    } else {
-      static const unsigned char fwd[4] = {'f', 'f', 't', 0};
-      if (!forward_dev(S, field, spec, fwd, st)) return -1;
+      c.kind = COL_C2C; c.fwd = fwd;
+      c.nld = c.n; c.ld_lo = c.n; c.ld_hi = c.n; c.ld_shift = 0;
+      c.nst = c.n; c.st_lo = c.n; c.st_shift = 0;
+      c.rev = rev;
+      launch_col(c, st);
+    };
+    launch_x_r2c(field, spec, (long long)ny * nz, nx, nxhp, nx, nxhp, st);
+    col(1, true, alt);
+    if (!(g_fused_solve && launch_zsolve(spec, nxhp, ny, nz, scale, st))) {
+      // composed: Z forward, the multiply pass, Z inverse
+      col(2, true, false);
+      const long long n = nxy * nz;
+      k_poisson<real_t><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(spec, nxhp, ny, nz, scale);
+      check_hip(hipGetLastError(), "k_poisson");
+      col(2, false, alt);
     }
-    const long long n = (long long)nc;
-    k_poisson<real_t><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(spec, S.nxhpc, ny, nz, scale);
-    check_hip(hipGetLastError(), "k_poisson");
-    static const unsigned char bwd[4] = {'t', 'f', 'f', 0};
-    if (!backward_dev(S, spec, field, bwd, st)) return -1;
-    S.w1 = nullptr;
+    col(1, false, false);
+    launch_x_c2r(spec, field, (long long)ny * nz, nxhp, nx, nx, nxhp, st, alt);
     return 0;
   } catch (const Error& e) {
     set_error(e.msg);

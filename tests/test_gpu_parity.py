@@ -442,3 +442,23 @@ def test_wide_offsets_bit_identical(precision, shape, ops):
             P.clean()
             P.lib.p3dfft_amd_set_wide_offsets(0)
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape,op", [((64, 48, 40), "fft"), ((30, 18, 22), "ffc"), ((36, 20, 30), "ffn")])
+def test_x_into_out_bit_identical(precision, shape, op):
+    """X r2c straight into the output with Y in place there computes every
+    line as the work-buffer schedule does: identical spectra."""
+    nx, ny, nz = shape
+    f = np.random.default_rng(29).standard_normal((nz, ny, nx))
+    res = []
+    for on in (1, 0):
+        P = p3dfft_amd.P3DFFT(precision)
+        P.lib.p3dfft_amd_set_x_into_out(on)
+        P.setup((1, 1), nx, ny, nz)
+        try:
+            res.append(P.ftran_r2c(f, op))
+        finally:
+            P.clean()
+            P.lib.p3dfft_amd_set_x_into_out(1)
+    assert np.array_equal(res[0], res[1])
