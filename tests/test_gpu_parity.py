@@ -283,6 +283,33 @@ def test_cheby_random_polynomial(precision, stride1):
         assert rel(P.cheby(a, Lz), o.cheby(a, Lz)) < cheby_tol(precision, nz, Lz)
 
 
+def test_c4_cheby_full_size_fp64():
+    """The Chebyshev configuration at its full size, 256 x 256 x 257 FP64: a
+    degree-64 Chebyshev polynomial in z (interpolated exactly on the 257
+    Gauss-Lobatto points) times trig modes in x, y.  The GPU result against the
+    oracle, and the driver_cheby round trip (end coefficients doubled, halved,
+    btran 'cff') against numpy's chebder of the polynomial."""
+    nx, ny, nz, Lz = 256, 256, 257, 2.0
+    rng = np.random.default_rng(45)
+    c = rng.standard_normal(65) / (1 + np.arange(65)) ** 2
+    zz = cheby_points(nz, Lz)
+    g = np.polynomial.chebyshev.chebval(zz, c)
+    dg = np.polynomial.chebyshev.chebval(zz, np.polynomial.chebyshev.chebder(c))
+    xs = np.arange(nx) / nx
+    ys = np.arange(ny) / ny
+    xy = np.cos(2 * np.pi * (3 * xs[None, :] + 5 * ys[:, None])) + 0.25
+    a = g[:, None, None] * xy[None]
+    o = oplan(nx, ny, nz)
+    with lib_setup(8, nx, ny, nz) as P:
+        got = P.cheby(a, Lz)
+        assert rel(got, o.cheby(a, Lz)) < TOL[8]
+        got[0] *= 2.0
+        got[nz - 1] *= 2.0
+        got *= 0.5
+        d = P.btran_c2r(got, "cff")
+    assert rel(d, dg[:, None, None] * xy[None]) < 1e-10
+
+
 # ---------------------------------------------------------------- errors, timers
 def test_error_paths():
     P = p3dfft_amd.P3DFFT(8)
