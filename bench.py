@@ -168,6 +168,55 @@ def fp64_poisson(f):
     return np.fft.irfftn(F * mult, s=(nz, ny, nx), axes=(0, 1, 2))
 
 
+def share_leg(steps=20, warmup=3):
+    """One rank of the 2048^3 transform on a 16 x 16 grid (rank (15, 3): local X
+    pencil (2048, 128, 128), 65 kx in its Y/Z pencils), FP32: the per-rank
+    stages of a fwd+bwd with the exchange buffers in HBM (the exchange itself
+    is another rank's business).  Per stage: mean time and algorithmic GB/s."""
+    n, dims, ip, jp = 2048, (16, 16), 15, 3
+    S = p3dfft_amd.Share(4)
+    S.setup_share(dims, ip, jp, n, n, n)
+    L = S.lib
+    try:
+        c = 8  # bytes per FP32 complex
+        nb1 = max(S.n_send1(), S.n_ypen()) * c
+        nb2 = max(S.n_send2(), S.n_zpen(), S.n_spec()) * c
+        inp = S.alloc(S.n_real() * 4); out = S.alloc(S.n_real() * 4)
+        b1 = S.alloc(nb1); b2 = S.alloc(nb2); b3 = S.alloc(S.n_zpen() * c)
+        for b in (inp, out, b1, b2, b3):
+            L.p3dfft_amd_memset(b.ptr, 0, b.nbytes)
+        stages = [("ftran_x", lambda: S.ftran_x(inp, b1), S.n_real() * 4 + S.n_send1() * c),
+                  ("ftran_y", lambda: S.ftran_y(b1, b2), (S.n_ypen() + S.n_send2()) * c),
+                  ("ftran_z", lambda: S.ftran_z(b2, b2, "fft"), (S.n_zpen() + S.n_spec()) * c),
+                  ("btran_z", lambda: S.btran_z(b2, b3, "tff"), (S.n_spec() + S.n_zpen()) * c),
+                  ("btran_y", lambda: S.btran_y(b3, b1), (S.n_zpen() + S.n_ypen()) * c),
+                  ("btran_x", lambda: S.btran_x(b1, out), S.n_send1() * c + S.n_real() * 4)]
+        for _ in range(warmup):
+            for _, f, _ in stages:
+                f()
+        ev = [L.p3dfft_amd_event_create() for _ in range(len(stages) + 1)]
+        acc = [0.0] * len(stages)
+        for _ in range(steps):
+            L.p3dfft_amd_event_record(ev[0], None)
+            for i, (_, f, _) in enumerate(stages):
+                f()
+                L.p3dfft_amd_event_record(ev[i + 1], None)
+            L.p3dfft_amd_synchronize()
+            for i in range(len(stages)):
+                acc[i] += L.p3dfft_amd_event_elapsed_ms(ev[i], ev[i + 1])
+        for e in ev:
+            L.p3dfft_amd_event_destroy(e)
+        for b in (inp, out, b1, b2, b3):
+            b.free()
+    finally:
+        S.clean()
+    st = {name: {"ms": acc[i] / steps, "bytes": nbytes, "GBps": nbytes / (acc[i] / steps * 1e-3) / 1e9}
+          for i, (name, _, nbytes) in enumerate(stages)}
+    return {"config": "rank (15,3) of 2048^3 on a 16x16 grid, local X pencil (2048,128,128), FP32; "
+                      "stages include the reference's pack/unpack maps, not the exchange",
+            "ms_per_fwd_bwd": sum(acc) / steps, "stages": st}
+
+
 def serve_leg(n_req, slots, load_frac=0.7):
     """Mixed-size spectral Poisson solves (grid sizes drawn per axis from
     {32,...,384}), FP32 with compensated twiddles, fields resident in HBM.
@@ -250,6 +299,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests of the served Poisson leg (0 = skip)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--no-share", action="store_true", help="skip the share-mode (one rank of 2048^3) leg")
     ap.add_argument("--field", choices=["modes", "random"], default="modes",
                     help="modes: the 8-mode + noise field; random: N(0,1) (fast, for profiler runs)")
     args = ap.parse_args()
@@ -317,6 +367,10 @@ def main():
         line["rel_l2_spectrum_fp32_vs_fp64"] = float(np.linalg.norm(r32["spec"] - ref) / np.linalg.norm(ref))
         b64 = r64["back"].astype(np.float64)
         line["rel_l2_roundtrip_fp32_vs_fp64"] = float(np.linalg.norm(r32["back"] - b64) / np.linalg.norm(b64))
+    if not args.no_share:
+        log("share-mode leg (one rank of 2048^3 on 16x16)")
+        line["share"] = share_leg()
+        log("share %.3f ms per rank fwd+bwd" % line["share"]["ms_per_fwd_bwd"])
     if args.serve_requests > 0:
         log("serve leg (%d requests)" % args.serve_requests)
         line["serve"] = serve_leg(args.serve_requests, args.serve_slots)

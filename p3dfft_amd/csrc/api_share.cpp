@@ -24,8 +24,15 @@ static void share_free() {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
+  share_free_tables(g.sg);
   if (g.w3) (void)hipFree(g.w3);
   g.w3 = nullptr;
+}
+
+static std::vector<int> zero_based(const std::vector<int>& st1) {
+  std::vector<int> v(st1.size());
+  for (size_t i = 0; i < v.size(); i++) v[i] = st1[i] - 1;
+  return v;
 }
 
 static int* upload_starts(const std::vector<int>& st1) {
@@ -109,6 +116,7 @@ int p3dfft_setup_share(int* dims, int* ipid, int* jpid, int* nx, int* ny, int* n
   s.d_iist = upload_starts(g.iist); s.d_iisz = upload(g.iisz);
   s.d_jist = upload_starts(g.jist); s.d_jisz = upload(g.jisz);
   s.d_jjst = upload_starts(g.jjst); s.d_jjsz = upload(g.jjsz);
+  share_build_tables(s, zero_based(g.iist), g.iisz, zero_based(g.jist), g.jisz, zero_based(g.jjst), g.jjsz);
   // work: w1 = buf2 (nxhp, jisize, kjsize); w2 = Y pencil (iisize, ny, kjsize);
   // w3 = Z pencil (iisize, jjsize, nz)
   check_hip(hipMalloc(&g.w1, std::max<size_t>(1, (size_t)g.nxhp * g.jisize * g.kjsize) * sizeof(cplx_t)), "hipMalloc");

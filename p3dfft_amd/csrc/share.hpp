@@ -5,6 +5,8 @@
 // to the caller, who receives the exact send buffers the reference would pass
 // to MPI and hands back the receive buffers.
 #pragma once
+#include <vector>
+
 #include "engine.hpp"
 
 namespace p3amd {
@@ -21,7 +23,24 @@ struct ShareGeom {
   int* d_jisz = nullptr;
   int* d_jjst = nullptr;  // nyc over p2
   int* d_jjsz = nullptr;
+  // Per-row address tables of the six maps (built once by share_build_tables).
+  // An element of a peer block is  table offset + outer index * table stride
+  // + inner index, so the kernels stream contiguous runs with no search:
+  long long* d_xo = nullptr;   // [nxhpc] fcomm1 send / bcomm2 recv: block start - x start
+  int* d_xs = nullptr;         // [nxhpc] run length (the owning row peer's x count)
+  long long* d_ya = nullptr;   // [ny] fcomm1 recv / bcomm2 send: offset of row y in its block
+  long long* d_yb = nullptr;   // [ny] z stride of that block
+  long long* d_ha = nullptr;   // [nyc] fcomm2 send / bcomm1 recv: offset of position yh
+  long long* d_hb = nullptr;   // [nyc] z stride of that block
+  int* d_hy = nullptr;         // [nyc] source row of position yh, -1 where left unwritten
+  int* d_yh = nullptr;         // [ny] position yh that fills row y, -1 for a zero row
 };
+
+// Build (and upload) the address tables from the 0-based splits.
+void share_build_tables(ShareGeom& s, const std::vector<int>& iist, const std::vector<int>& iisz,
+                        const std::vector<int>& jist, const std::vector<int>& jisz, const std::vector<int>& jjst,
+                        const std::vector<int>& jjsz);
+void share_free_tables(ShareGeom& s);
 
 void share_pack_fcomm1(const ShareGeom& s, const cplx_t* buf2, cplx_t* send, hipStream_t st);
 void share_unpack_fcomm1(const ShareGeom& s, const cplx_t* recv, cplx_t* dest, hipStream_t st);
