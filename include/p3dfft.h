@@ -128,6 +128,10 @@ int p3dfft_setup_share(int *dims, int *ipid, int *jpid, int *nx, int *ny, int *n
                        int *ow, int *memsize);
 /* per-peer offsets/counts (complex elements): forward row exchange (p1 peers),
  * forward column exchange (p2 peers); backward exchanges swap send and recv */
+/* 1 (default): the share-mode stages fuse the pack/unpack maps into the FFT
+ * kernels' stores/loads where the lengths have specialised kernels; 0: separate
+ * pack/unpack passes.  The buffers are identical either way. */
+void p3dfft_share_set_fused(int on);
 int p3dfft_share_counts(long long *fsnd_st, long long *fsnd_ct, long long *frcv_st, long long *frcv_ct,
                         long long *ksnd_st, long long *ksnd_ct, long long *krcv_st, long long *krcv_ct);
 /* which: 0 fcomm1 pack, 1 fcomm1 unpack, 2 fcomm2 pack, 3 bcomm1 unpack,

@@ -53,6 +53,10 @@ static int* upload(const std::vector<int>& v) {
 
 void share_release() { share_free(); }
 
+// pack/unpack fused into the FFT kernels' loads and stores where the lengths
+// have specialised kernels (default); off: separate pack/unpack passes
+static bool g_share_fused = true;
+
 static bool require_share() {
   if (!require_setup()) return false;
   if (!g.share) return fatal(" p3dfft_amd: the share-mode stages need p3dfft_setup_share");
@@ -138,6 +142,10 @@ int p3dfft_setup_share(int* dims, int* ipid, int* jpid, int* nx, int* ny, int* n
 // exchange buffers: forward row (p1 entries), forward column (p2 entries); the
 // backward exchanges use the same blocks with send and receive swapped
 // (build/setup.F90:480-518).
+// 1 (default): share-mode stages fuse the pack/unpack maps into the FFT
+// kernels where they can; 0: separate pack/unpack passes (same buffers).
+void p3dfft_share_set_fused(int on) { g_share_fused = (on != 0); }
+
 int p3dfft_share_counts(long long* fsnd_st, long long* fsnd_ct, long long* frcv_st, long long* frcv_ct,
                         long long* ksnd_st, long long* ksnd_ct, long long* krcv_st, long long* krcv_ct) {
   if (!require_share()) return -1;
@@ -184,6 +192,14 @@ int p3dfft_share_ftran_x(const p3amd::real_t* in, p3amd::real_t* send1, void* st
   if (!require_share()) return -1;
   GUARD_BEGIN
   hipStream_t st = (hipStream_t)stream;
+  tstart(g, 5, st);
+  // the X kernel writes the send buffer through the fcomm1 map when the
+  // length has a specialised kernel; otherwise X into buf2, then the pack
+  const bool fused = g.sg.d_xt && g_share_fused &&
+                     launch_x_r2c_tab(in, (cplx_t*)send1, (long long)g.jisize * g.kjsize, g.nx, g.nxhpc,
+                                      g.sg.d_xt, st);
+  tstop(g, 5, st);
+  if (fused) return 0;
   tstart(g, 5, st);
   launch_x_r2c(in, g.w1, (long long)g.jisize * g.kjsize, g.nx, g.nxhp, g.nx, g.nxhpc, st);
   tstop(g, 5, st);
@@ -317,6 +333,12 @@ int p3dfft_share_btran_x(const p3amd::real_t* recv, p3amd::real_t* out, void* st
   if (!require_share()) return -1;
   GUARD_BEGIN
   hipStream_t st = (hipStream_t)stream;
+  tstart(g, 12, st);
+  const bool fused = g.sg.d_xt && g_share_fused &&
+                     launch_x_c2r_tab((const cplx_t*)recv, out, (long long)g.jisize * g.kjsize, g.nx, g.nxhpc,
+                                      g.sg.d_xt, st);
+  tstop(g, 12, st);
+  if (fused) return 0;
   tstart(g, 11, st);
   share_unpack_bcomm2(g.sg, (const cplx_t*)recv, g.w1, st);
   tstop(g, 11, st);

@@ -195,6 +195,9 @@ struct XArgs {
   int tw_lds;  // twiddles staged in LDS
   int tw_off;  // byte offset of the staged twiddles
   int rev;     // walk the tiles from the last one
+  // table-addressed rows (share mode): mode k of row r at element
+  // xt[k].x + r * xt[k].y of `out` (r2c) / `in` (c2r), for k < nxo
+  const int2* xt;
 };
 
 extern __shared__ __align__(16) unsigned char g_smem[];
@@ -358,7 +361,7 @@ __device__ __forceinline__ const typename TwT<T>::type* ct_twiddles(const void* 
   else return (const TwL*)tw;
 }
 
-template <class T, int B, int NT, class RL, int W, bool TWL>
+template <class T, int B, int NT, class RL, int W, bool TWL, bool XT>
 __global__ __launch_bounds__(NT, W) void k_x_r2c_ct(XArgs a) {
   constexpr int N = RL_N<RL>::value;
   constexpr int BP = B + 1;
@@ -404,7 +407,13 @@ __global__ __launch_bounds__(NT, W) void k_x_r2c_ct(XArgs a) {
     const cpx<T> zk = lds[k * BP + b], zn = lds[(k == 0 ? 0 : N - k) * BP + b];
     const unsigned o = (unsigned)(2 * b) * so + (unsigned)k * (unsigned)sizeof(cpx<T>);
     const cpx<T> ya = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h), yb = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
-    if (full) {
+    if constexpr (XT) {
+      const int2 tk = a.xt[k];
+      const long long r = 2 * (pair0 + b);
+      cpx<T>* oc = (cpx<T>*)a.out;
+      if (2 * b < nr) oc[tk.x + r * tk.y] = ya;
+      if (2 * b + 1 < nr) oc[tk.x + (r + 1) * tk.y] = yb;
+    } else if (full) {
       *(cpx<T>*)(out + o) = ya;
       *(cpx<T>*)(out + o + so) = yb;
     } else {
@@ -414,7 +423,7 @@ __global__ __launch_bounds__(NT, W) void k_x_r2c_ct(XArgs a) {
   }
 }
 
-template <class T, int B, int NT, class RL, int W, bool TWL>
+template <class T, int B, int NT, class RL, int W, bool TWL, bool XT>
 __global__ __launch_bounds__(NT, W) void k_x_c2r_ct(XArgs a) {
   constexpr int N = RL_N<RL>::value;
   constexpr int BP = B + 1;
@@ -439,11 +448,21 @@ __global__ __launch_bounds__(NT, W) void k_x_c2r_ct(XArgs a) {
       // out-of-range items load the tile's first element (always valid) and
       // are zeroed, so no load sits under a branch
       const bool oka = ok && 2 * b < nr, okb = ok && 2 * b + 1 < nr;
-      const unsigned o = (unsigned)(2 * b) * si + (unsigned)k * (unsigned)sizeof(cpx<T>);
-      const cpx<T> ua = *(const cpx<T>*)(in + (oka ? o : 0u));
-      const cpx<T> ub = *(const cpx<T>*)(in + (okb ? o + si : 0u));
-      va[i] = oka ? ua : mk<T>(0, 0);
-      vb[i] = okb ? ub : mk<T>(0, 0);
+      if constexpr (XT) {
+        const cpx<T>* ic = (const cpx<T>*)a.in;
+        const int2 tk = a.xt[ok ? k : 0];
+        const long long r = 2 * (pair0 + b);
+        const cpx<T> ua = ic[oka ? tk.x + r * tk.y : 0];
+        const cpx<T> ub = ic[okb ? tk.x + (r + 1) * tk.y : 0];
+        va[i] = oka ? ua : mk<T>(0, 0);
+        vb[i] = okb ? ub : mk<T>(0, 0);
+      } else {
+        const unsigned o = (unsigned)(2 * b) * si + (unsigned)k * (unsigned)sizeof(cpx<T>);
+        const cpx<T> ua = *(const cpx<T>*)(in + (oka ? o : 0u));
+        const cpx<T> ub = *(const cpx<T>*)(in + (okb ? o + si : 0u));
+        va[i] = oka ? ua : mk<T>(0, 0);
+        vb[i] = okb ? ub : mk<T>(0, 0);
+      }
     }
 #pragma unroll
     for (int i = 0; i < LI; i++) {
@@ -866,10 +885,20 @@ constexpr int ct_waves(long long lds_bytes, int nt) {
   const long long w = wg * (nt / 64) / 4;
   return w < 1 ? 1 : (w > 4 ? 4 : (int)w);  // at least 128 registers per lane
 }
+// X kernels: up to 16 row pairs per tile, 16 complex values per thread, at
+// most 512 threads and an 80 KiB padded tile (two workgroups per CU)
+#ifndef P3DFFT_X_EPT
+#define P3DFFT_X_EPT 16
+#endif
+constexpr int kEPTx = P3DFFT_X_EPT;
 constexpr int ct_x_pairs(int N) {
   int b = 16;
-  while (b > 1 && (b * N > 256 * kEPTct || (long long)(b + 1) * N * (long long)sizeof(cplx_t) > 72 * 1024)) b /= 2;
+  while (b > 1 && (b * N > 512 * kEPTx || (long long)(b + 1) * N * (long long)sizeof(cplx_t) > 80 * 1024)) b /= 2;
   return b;
+}
+constexpr int ct_nt_x(int elems) {
+  int nt = (elems / kEPTx + 63) / 64 * 64;
+  return nt < 64 ? 64 : (nt > 512 ? 512 : nt);
 }
 
 // Twiddles of a specialised length sit in LDS behind the data tile when the
@@ -896,7 +925,7 @@ struct CtLen {
   static constexpr int TX = ct_col_tx(N);
   static constexpr int NTC = ct_nt(TX * N);
   static constexpr int BX = ct_x_pairs(N);
-  static constexpr int NTX = ct_nt(BX * N);
+  static constexpr int NTX = ct_nt_x(BX * N);
   static constexpr long long TWB = (long long)TwCount<Rs...>::value() * sizeof(Tw);
   static constexpr long long DC = round16((long long)TX * N * sizeof(cplx_t));
   static constexpr long long DX = round16((long long)(BX + 1) * N * sizeof(cplx_t));
@@ -936,10 +965,12 @@ struct CtLen {
     }
   }
   static void xr2c(const XArgs& a, unsigned g, size_t bytes, hipStream_t s) {
-    gox(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX>, g, NTX, bytes, s, a);
+    if (a.xt) gox(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX, true>, g, NTX, bytes, s, a);
+    else gox(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX, false>, g, NTX, bytes, s, a);
   }
   static void xc2r(const XArgs& a, unsigned g, size_t bytes, hipStream_t s) {
-    gox(k_x_c2r_ct<real_t, BX, NTX, RL, WX, TWLX>, g, NTX, bytes, s, a);
+    if (a.xt) gox(k_x_c2r_ct<real_t, BX, NTX, RL, WX, TWLX, true>, g, NTX, bytes, s, a);
+    else gox(k_x_c2r_ct<real_t, BX, NTX, RL, WX, TWLX, false>, g, NTX, bytes, s, a);
   }
   static void zsolve(const ColArgs& a, double scale, int nxh, int ny, unsigned g, size_t bytes, hipStream_t s) {
     auto k = k_zsolve_ct<real_t, TX, NTC, RL, WC, TWLC>;
@@ -1057,6 +1088,42 @@ void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_r
   x_setup(a, L, nrows, nx, &bytes, &nt, &nwg);
   dispatch_nt<GoXr2c>(nt, a, nwg, bytes, st);
   check_hip(hipGetLastError(), "k_x_r2c launch");
+}
+
+bool launch_x_r2c_tab(const real_t* in, cplx_t* out, long long nrows, int nx, int nxo, const int2* xt,
+                      hipStream_t st) {
+  if (nrows <= 0 || nxo <= 0) return true;
+  const Len1D& L = get_len(nx);
+  const CtInfo* c = g_use_ct ? find_ct(L) : nullptr;
+  if (!c) return false;
+  XArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in; a.out = out; a.nrows = nrows; a.si_row = nx; a.so_row = 0;
+  a.nxo = nxo; a.dnxo = make_fdiv(nxo);
+  a.xt = xt;
+  size_t bytes; unsigned nwg;
+  if (!x_setup_ct(a, L, *c, nrows, nx, 1, &bytes, &nwg)) return false;
+  c->xr2c(a, nwg, bytes, st);
+  check_hip(hipGetLastError(), "k_x_r2c_ct (table) launch");
+  return true;
+}
+
+bool launch_x_c2r_tab(const cplx_t* in, real_t* out, long long nrows, int nx, int nxi, const int2* xt,
+                      hipStream_t st) {
+  if (nrows <= 0) return true;
+  const Len1D& L = get_len(nx);
+  const CtInfo* c = g_use_ct ? find_ct(L) : nullptr;
+  if (!c) return false;
+  XArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in; a.out = out; a.nrows = nrows; a.si_row = 0; a.so_row = nx;
+  a.nxo = nxi; a.dnxo = make_fdiv(nxi > 0 ? nxi : 1);
+  a.xt = xt;
+  size_t bytes; unsigned nwg;
+  if (!x_setup_ct(a, L, *c, nrows, 1, nx, &bytes, &nwg)) return false;
+  c->xc2r(a, nwg, bytes, st);
+  check_hip(hipGetLastError(), "k_x_c2r_ct (table) launch");
+  return true;
 }
 
 void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_row, long long so_row, int nx,

@@ -52,6 +52,13 @@ void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_r
                   int nx, int nxo, hipStream_t st, bool rev = false);
 void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_row, long long so_row,
                   int nx, int nxi, hipStream_t st, bool rev = false);
+// The same with table-addressed complex rows: mode k of row r at element
+// xt[k].x + r * xt[k].y (k < nxo / nxi).  False when nx has no specialised
+// kernel (the caller then goes through a contiguous buffer).
+bool launch_x_r2c_tab(const real_t* in, cplx_t* out, long long nrows, int nx, int nxo, const int2* xt,
+                      hipStream_t st);
+bool launch_x_c2r_tab(const cplx_t* in, real_t* out, long long nrows, int nx, int nxi, const int2* xt,
+                      hipStream_t st);
 
 // Column transform along a strided axis of a 3D view [outer][len][inner].
 enum ColKind { COL_C2C = 0, COL_DCT1 = 1, COL_DST1 = 2, COL_COPY = 3 };

@@ -332,11 +332,16 @@ void share_build_tables(ShareGeom& s, const std::vector<int>& iist, const std::v
   s.d_ya = upload_vec(ya); s.d_yb = upload_vec(yb);
   s.d_ha = upload_vec(ha); s.d_hb = upload_vec(hb); s.d_hy = upload_vec(hy);
   s.d_yh = upload_vec(yh_of);
+  if ((long long)s.nxhpc * unit_x < (1LL << 31)) {
+    std::vector<int2> xt(s.nxhpc);
+    for (int x = 0; x < s.nxhpc; x++) xt[x] = make_int2((int)(xo[x] + x), xs[x]);
+    s.d_xt = upload_vec(xt);
+  }
 }
 
 void share_free_tables(ShareGeom& s) {
   for (void** p : {(void**)&s.d_xo, (void**)&s.d_xs, (void**)&s.d_ya, (void**)&s.d_yb, (void**)&s.d_ha,
-                   (void**)&s.d_hb, (void**)&s.d_hy, (void**)&s.d_yh}) {
+                   (void**)&s.d_hb, (void**)&s.d_hy, (void**)&s.d_yh, (void**)&s.d_xt}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }

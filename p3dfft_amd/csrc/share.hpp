@@ -34,6 +34,10 @@ struct ShareGeom {
   long long* d_hb = nullptr;   // [nyc] z stride of that block
   int* d_hy = nullptr;         // [nyc] source row of position yh, -1 where left unwritten
   int* d_yh = nullptr;         // [ny] position yh that fills row y, -1 for a zero row
+  // [nxhpc] {xo + x, xs} as 32-bit pairs for the X kernels that write the
+  // fcomm1 send buffer / read the bcomm2 receive buffer directly (null when
+  // those buffers exceed 2^31 elements)
+  int2* d_xt = nullptr;
 };
 
 // Build (and upload) the address tables from the 0-based splits.

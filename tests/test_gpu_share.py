@@ -201,6 +201,33 @@ def test_all_ranks_stage_by_stage(precision, dims):
     assert rel(field, a * nx * ny * nz) < TOL[precision] * 10
 
 
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("dims", [(2, 3), (4, 2)])
+def test_all_ranks_fused_stages(precision, dims):
+    """Lengths with specialised kernels (64, 48, 32): the X stages write the
+    fcomm1 send buffer / read the bcomm2 receive buffer through the maps
+    directly.  Results match the single-rank transform and are bit-identical
+    to the separate pack/unpack passes."""
+    nx, ny, nz = 64, 48, 32
+    a = np.random.default_rng(6).standard_normal((nz, ny, nx))
+    runs = []
+    for fused in (1, 0):
+        p3dfft_amd.load(precision).p3dfft_share_set_fused(fused)
+        try:
+            runs.append(run_grid_on_gpu(precision, dims, nx, ny, nz, a))
+        finally:
+            p3dfft_amd.load(precision).p3dfft_share_set_fused(1)
+    (spec, field, _), (spec0, field0, _) = runs
+    full = O.Plan(nx, ny, nz).ftran_r2c(a)
+    for (i, j), s in spec.items():
+        o = O.Plan(nx, ny, nz, dims=dims, ipid=i, jpid=j)
+        ref = full[:, o.jjstart - 1:o.jjend, o.iistart - 1:o.iiend].reshape(-1)
+        assert rel(s, ref) < TOL[precision], (i, j)
+        assert np.array_equal(s, spec0[(i, j)]), (i, j)
+    assert rel(field, a * nx * ny * nz) < TOL[precision] * 10
+    assert np.array_equal(field, field0)
+
+
 def test_one_rank_of_2048cube_on_16x16_fp32():
     """Rank (15, 3) of 2048^3 on 16x16 (local X pencil (2048,128,128), the last
     row rank holds 65 kx): each stage's send buffer against numpy FFTs and the
