@@ -216,6 +216,17 @@ int p3dfft_share_ftran_y(const p3amd::real_t* recv1, p3amd::real_t* send2, void*
   if (!require_share()) return -1;
   GUARD_BEGIN
   hipStream_t st = (hipStream_t)stream;
+  if (g_share_fused && g.sg.d_yt) {
+    // the Y kernel reads the fcomm1 receive buffer and writes the fcomm2
+    // send buffer through the maps (row tables)
+    ColSpec y{};
+    y.in = (const cplx_t*)recv1; y.out = (cplx_t*)send2;
+    y.n_inner = g.iisize; y.n_outer = g.kjsize; y.n = g.ny; y.kind = COL_C2C; y.fwd = true;
+    tstart(g, 7, st);
+    const bool done = launch_col_tab(y, g.sg.d_yt, g.sg.d_ysf, st);
+    tstop(g, 7, st);
+    if (done) return 0;
+  }
   tstart(g, 6, st);
   share_unpack_fcomm1(g.sg, (const cplx_t*)recv1, g.w2, st);
   tstop(g, 6, st);
@@ -307,6 +318,17 @@ int p3dfft_share_btran_y(const p3amd::real_t* recv, p3amd::real_t* send, void* s
   if (!require_share()) return -1;
   GUARD_BEGIN
   hipStream_t st = (hipStream_t)stream;
+  if (g_share_fused && g.sg.d_yt) {
+    // the Y kernel reads the bcomm1 receive buffer (zero rows included) and
+    // writes the bcomm2 send buffer through the maps (row tables)
+    ColSpec y{};
+    y.in = (const cplx_t*)recv; y.out = (cplx_t*)send;
+    y.n_inner = g.iisize; y.n_outer = g.kjsize; y.n = g.ny; y.kind = COL_C2C; y.fwd = false;
+    tstart(g, 10, st);
+    const bool done = launch_col_tab(y, g.sg.d_ylb, g.sg.d_yt, st);
+    tstop(g, 10, st);
+    if (done) return 0;
+  }
   tstart(g, 9, st);
   share_unpack_bcomm1(g.sg, (const cplx_t*)recv, g.w2, st);
   tstop(g, 9, st);

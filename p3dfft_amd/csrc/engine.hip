@@ -285,6 +285,8 @@ struct ColArgs {
   const void* tw;
   int tw_lds, tw_off;
   int rev;  // walk the tiles from the last one
+  const int2* ltab;  // table-addressed rows (see ColIO)
+  const int2* stab;
 };
 
 template <class T, bool FWD, int NT, int EPT>
@@ -518,6 +520,12 @@ __device__ __forceinline__ ColIO col_io(const ColArgs& a, long long c0, int nb) 
   io.nb = nb;
   io.ld_lo = a.ld_lo; io.ld_hi = a.ld_hi; io.ld_shift = a.ld_shift;
   io.nst = a.nst; io.st_lo = a.st_lo; io.st_shift = a.st_shift;
+  io.lane_o = o; io.lane_i = i;
+  io.ltab = a.ltab; io.stab = a.stab;
+  if (a.ltab) {  // table-addressed rows index the whole arrays
+    io.in = (const char*)a.in;
+    io.out = (char*)a.out;
+  }
   return io;
 }
 
@@ -950,10 +958,16 @@ struct CtLen {
     if (fwd) go(k_col_ct<real_t, true, TX, NTC, RL, WC, TWLC>, g, NTC, bytes, s, a);
     else go(k_col_ct<real_t, false, TX, NTC, RL, WC, TWLC>, g, NTC, bytes, s, a);
   }
-  // pr: pruning maps; wide: 64-bit offsets (unpruned views only)
+  // pr: pruning maps; wide: 64-bit offsets (unpruned views only); a.ltab:
+  // table-addressed rows
   static void colio(bool fwd, bool pr, bool wide, const ColArgs& a, unsigned g, size_t bytes, hipStream_t s) {
     typedef unsigned U32;
     typedef unsigned long long U64;
+    if (a.ltab) {
+      if (fwd) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, false, TabRows>, g, NTC, bytes, s, a);
+      else go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, false, TabRows>, g, NTC, bytes, s, a);
+      return;
+    }
     if (fwd) {
       if (pr) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, true, U32>, g, NTC, bytes, s, a);
       else if (wide) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, false, U64>, g, NTC, bytes, s, a);
@@ -1158,6 +1172,34 @@ static bool col_fits32(const ColSpec& c, int TX, int rows) {
     return (nout * so + (long long)c.n_inner * si + (long long)rows * sl + 1) * (long long)sizeof(cplx_t);
   };
   return span(c.si_in, c.sl_in, c.so_in) < (1LL << 32) && span(c.si_out, c.sl_out, c.so_out) < (1LL << 32);
+}
+
+bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStream_t st) {
+  if (c.n_inner <= 0 || c.n_outer <= 0) return true;
+  if (c.kind != COL_C2C || !g_use_ct) return false;
+  const Len1D& L = get_len(c.n);
+  const CtInfo* ct = find_ct(L);
+  if (!ct) return false;
+  const long long ncol = (long long)c.n_inner * c.n_outer;
+  if (ncol > 0x7fffffffLL) return false;
+  ColArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = c.in; a.out = c.out;
+  a.n_inner = c.n_inner; a.n_outer = c.n_outer;
+  a.kind = COL_C2C;
+  a.ltab = ltab; a.stab = stab;
+  a.rev = c.rev ? 1 : 0;
+  const int TX = ct->TX;
+  a.TX = TX;
+  a.dTX = make_fdiv(TX);
+  a.tiles = (c.n_inner + TX - 1) / TX;
+  a.L = L.d;
+  a.tw = L.tw;
+  const size_t bytes = ct_lds((size_t)TX * c.n * sizeof(cplx_t), ct->twl_col, *ct, &a.tw_lds, &a.tw_off);
+  if (bytes > kLdsMax) return false;
+  ct->colio(c.fwd, false, false, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
+  check_hip(hipGetLastError(), "k_colio_ct (table) launch");
+  return true;
 }
 
 void launch_col(const ColSpec& c, hipStream_t st) {

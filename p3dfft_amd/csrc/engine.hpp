@@ -78,6 +78,12 @@ struct ColSpec {
   bool rev;       // walk the tiles from the last one (see set_tile_order)
 };
 void launch_col(const ColSpec& c, hipStream_t st);
+// C2C column transform of n_inner x n_outer columns of length n whose rows
+// are table-addressed: row e of column (o, i) at element ltab[e].x + o *
+// ltab[e].y + i of c.in (zero when ltab[e].x < 0), FFT row p stored at
+// stab[p] likewise in c.out (skipped when < 0).  False when n has no
+// specialised kernel.
+bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStream_t st);
 
 // Consecutive passes over one buffer walk their tiles in opposite orders, so
 // a pass starts on what the previous pass wrote last (still in the

@@ -10,6 +10,7 @@
 // table layout as the run-time engine (stage 0 has none; stage s holds
 // Ns*(R-1) entries W_{Ns R}^{k r}, k < Ns, r = 1..R-1).
 #pragma once
+#include <type_traits>
 #include "fft_device.hpp"
 
 namespace p3amd {
@@ -96,7 +97,16 @@ struct ColIO {
   int nb;                     // valid columns of this tile
   int ld_lo, ld_hi, ld_shift; // PR: LDS row e <- input row e (e<ld_lo), e-ld_shift (e>=ld_hi), else 0
   int nst, st_lo, st_shift;   // PR: output row e <- FFT row e (e<st_lo) or e+st_shift
+  // table-addressed rows (Off = TabRows): row e of this lane's column at
+  // element ltab[e].x + lane_o * ltab[e].y + lane_i of `in` (zero when
+  // ltab[e].x < 0), FFT row p stored at stab[p] likewise (skipped when < 0)
+  const int2* ltab;
+  const int2* stab;
+  unsigned lane_o, lane_i;    // this lane's outer and inner column index
 };
+
+// offset type selecting the table-addressed rows
+struct TabRows {};
 
 template <int R, int N, int B, int NT>
 struct Stage0 {
@@ -120,6 +130,7 @@ __device__ __forceinline__ void st_at(char* base, Off off, cpx<T> v) {
 template <class T, int R, bool FWD, int N, int B, int NT, bool PR, class Off>
 __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
   static_assert(NT % B == 0, "one column per thread");
+  constexpr bool TB = std::is_same<Off, TabRows>::value;
   typedef Stage0<R, N, B, NT> S;
   const bool valid = (int)(threadIdx.x % B) < io.nb;
   cpx<T> v[S::IT][R];
@@ -133,7 +144,13 @@ __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
         const int e = j + r * S::nR;
         // items outside the tile load its first element (always valid) and
         // are zeroed, so no load sits under a branch
-        if constexpr (PR) {
+        if constexpr (TB) {
+          const int2 t = io.ltab[e];
+          const bool ok = valid && t.x >= 0;
+          const unsigned el = ok ? (unsigned)t.x + io.lane_o * (unsigned)t.y + io.lane_i : 0u;
+          const cpx<T> u = ((const cpx<T>*)io.in)[el];
+          v[m][r] = ok ? u : mk<T>(0, 0);
+        } else if constexpr (PR) {
           const int src = e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1);
           const bool ok = valid && src >= 0;
           const cpx<T> u = ld_at<T, Off>(io.in, ok ? (Off)io.lane_in + (Off)src * (Off)io.sl_in : (Off)0);
@@ -164,6 +181,7 @@ __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename 
                                               const ColIO& io) {
   static_assert(Ns * R == N, "last stage");
   static_assert(NT % B == 0, "one column per thread");
+  constexpr bool TB = std::is_same<Off, TabRows>::value;
   constexpr int total = B * Ns;
   constexpr int IT = (total + NT - 1) / NT;
   const bool valid = (int)(threadIdx.x % B) < io.nb;
@@ -183,7 +201,10 @@ __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename 
 #pragma unroll
         for (int r = 0; r < R; r++) {
           const int p = j + r * Ns;
-          if constexpr (PR) {
+          if constexpr (TB) {
+            const int2 t = io.stab[p];
+            if (t.x >= 0) ((cpx<T>*)io.out)[(unsigned)t.x + io.lane_o * (unsigned)t.y + io.lane_i] = v[r];
+          } else if constexpr (PR) {
             const int e = p < io.st_lo ? p : p - io.st_shift;
             if (p < io.st_lo || (p >= io.st_lo + io.st_shift && e < io.nst))
               st_at<T, Off>(io.out, (Off)io.lane_out + (Off)e * (Off)io.sl_out, v[r]);

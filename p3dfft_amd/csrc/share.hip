@@ -332,6 +332,15 @@ void share_build_tables(ShareGeom& s, const std::vector<int>& iist, const std::v
   s.d_ya = upload_vec(ya); s.d_yb = upload_vec(yb);
   s.d_ha = upload_vec(ha); s.d_hb = upload_vec(hb); s.d_hy = upload_vec(hy);
   s.d_yh = upload_vec(yh_of);
+  if ((long long)s.iisize * s.ny * s.kjsize < (1LL << 31)) {
+    std::vector<int2> yt(s.ny), ysf(s.ny, make_int2(-1, 0)), ylb(s.ny, make_int2(-1, 0));
+    for (int y = 0; y < s.ny; y++) yt[y] = make_int2((int)ya[y], (int)yb[y]);
+    for (int yh = 0; yh < s.nyc; yh++)
+      if (hy[yh] >= 0) ysf[hy[yh]] = make_int2((int)ha[yh], (int)hb[yh]);
+    for (int y = 0; y < s.ny; y++)
+      if (yh_of[y] >= 0) ylb[y] = make_int2((int)ha[yh_of[y]], (int)hb[yh_of[y]]);
+    s.d_yt = upload_vec(yt); s.d_ysf = upload_vec(ysf); s.d_ylb = upload_vec(ylb);
+  }
   if ((long long)s.nxhpc * unit_x < (1LL << 31)) {
     std::vector<int2> xt(s.nxhpc);
     for (int x = 0; x < s.nxhpc; x++) xt[x] = make_int2((int)(xo[x] + x), xs[x]);
@@ -341,7 +350,8 @@ void share_build_tables(ShareGeom& s, const std::vector<int>& iist, const std::v
 
 void share_free_tables(ShareGeom& s) {
   for (void** p : {(void**)&s.d_xo, (void**)&s.d_xs, (void**)&s.d_ya, (void**)&s.d_yb, (void**)&s.d_ha,
-                   (void**)&s.d_hb, (void**)&s.d_hy, (void**)&s.d_yh, (void**)&s.d_xt}) {
+                   (void**)&s.d_hb, (void**)&s.d_hy, (void**)&s.d_yh, (void**)&s.d_xt, (void**)&s.d_yt,
+                   (void**)&s.d_ysf, (void**)&s.d_ylb}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }

@@ -38,6 +38,13 @@ struct ShareGeom {
   // fcomm1 send buffer / read the bcomm2 receive buffer directly (null when
   // those buffers exceed 2^31 elements)
   int2* d_xt = nullptr;
+  // [ny] row tables of the Y column kernel that reads / writes the exchange
+  // buffers directly: {ya, yb} (fcomm1 recv, bcomm2 send), the fcomm2 send
+  // position of FFT row p (-1: not sent) and the bcomm1 recv position that
+  // fills row y (-1: zero row); null when the buffers exceed 2^31 elements
+  int2* d_yt = nullptr;
+  int2* d_ysf = nullptr;
+  int2* d_ylb = nullptr;
 };
 
 // Build (and upload) the address tables from the 0-based splits.

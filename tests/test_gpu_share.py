@@ -204,10 +204,11 @@ def test_all_ranks_stage_by_stage(precision, dims):
 @pytest.mark.parametrize("precision", [8, 4])
 @pytest.mark.parametrize("dims", [(2, 3), (4, 2)])
 def test_all_ranks_fused_stages(precision, dims):
-    """Lengths with specialised kernels (64, 48, 32): the X stages write the
-    fcomm1 send buffer / read the bcomm2 receive buffer through the maps
-    directly.  Results match the single-rank transform and are bit-identical
-    to the separate pack/unpack passes."""
+    """Lengths with specialised kernels (64, 48, 32): the X and Y stages read
+    and write the exchange buffers through the maps directly.  Results match
+    the single-rank transform and the separate pack/unpack passes (to the
+    last bits: the fused kernels are other instantiations of the butterflies,
+    which the compiler may contract into FMAs differently)."""
     nx, ny, nz = 64, 48, 32
     a = np.random.default_rng(6).standard_normal((nz, ny, nx))
     runs = []
@@ -223,9 +224,9 @@ def test_all_ranks_fused_stages(precision, dims):
         o = O.Plan(nx, ny, nz, dims=dims, ipid=i, jpid=j)
         ref = full[:, o.jjstart - 1:o.jjend, o.iistart - 1:o.iiend].reshape(-1)
         assert rel(s, ref) < TOL[precision], (i, j)
-        assert np.array_equal(s, spec0[(i, j)]), (i, j)
+        assert rel(s, spec0[(i, j)]) < TOL[precision] / 10, (i, j)
     assert rel(field, a * nx * ny * nz) < TOL[precision] * 10
-    assert np.array_equal(field, field0)
+    assert rel(field, field0) < TOL[precision] / 10
 
 
 def test_one_rank_of_2048cube_on_16x16_fp32():
