@@ -287,6 +287,7 @@ struct ColArgs {
   int rev;  // walk the tiles from the last one
   const int2* ltab;  // table-addressed rows (see ColIO)
   const int2* stab;
+  int tab_off;       // byte offset of the LDS copy of the row tables
 };
 
 template <class T, bool FWD, int NT, int EPT>
@@ -541,7 +542,10 @@ __global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
   const long long c0 = (long long)tile_of(a.rev) * TX;
   const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
   const ColIO io = col_io<T, TX>(a, c0, nb);
-  run_fft_io<T, FWD, TX, NT, PR, Off>(lds, tw, io, RL{});
+  if constexpr (std::is_same<Off, TabRows>::value)
+    run_fft_io_tab<T, FWD, TX, NT>(lds, (int2*)(g_smem + a.tab_off), tw, io, a.ltab, a.stab, RL{});
+  else
+    run_fft_io<T, FWD, TX, NT, PR, Off>(lds, tw, io, RL{});
 }
 
 // Column transforms through an LDS load/store pass: DCT-I / DST-I
@@ -1195,7 +1199,9 @@ bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStr
   a.tiles = (c.n_inner + TX - 1) / TX;
   a.L = L.d;
   a.tw = L.tw;
-  const size_t bytes = ct_lds((size_t)TX * c.n * sizeof(cplx_t), ct->twl_col, *ct, &a.tw_lds, &a.tw_off);
+  size_t bytes = ct_lds((size_t)TX * c.n * sizeof(cplx_t), ct->twl_col, *ct, &a.tw_lds, &a.tw_off);
+  a.tab_off = (int)round16((long long)bytes);
+  bytes = (size_t)a.tab_off + (size_t)c.n * sizeof(int2);
   if (bytes > kLdsMax) return false;
   ct->colio(c.fwd, false, false, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
   check_hip(hipGetLastError(), "k_colio_ct (table) launch");

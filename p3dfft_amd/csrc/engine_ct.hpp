@@ -236,6 +236,24 @@ __device__ __forceinline__ void run_fft_io(cpx<T>* lds, const typename TwT<T>::t
   mid_and_last<T, FWD, N, B, NT, R0, 0, PR, Off, Rs...>(lds, tw, io);
 }
 
+// Table-addressed rows (Off = TabRows) with the row tables staged in LDS at
+// `tl` (n int2): the load table before the first stage, the store table
+// after it, so every table read is an LDS read.
+template <class T, bool FWD, int B, int NT, int R0, int... Rs>
+__device__ __forceinline__ void run_fft_io_tab(cpx<T>* lds, int2* tl, const typename TwT<T>::type* tw, ColIO io,
+                                               const int2* ltab, const int2* stab, RList<R0, Rs...>) {
+  static_assert(sizeof...(Rs) >= 1, "at least two stages");
+  constexpr int N = R0 * RProd<Rs...>::value;
+  for (int i = threadIdx.x; i < N; i += NT) tl[i] = ltab[i];
+  __syncthreads();
+  io.ltab = tl;
+  stage0_global<T, R0, FWD, N, B, NT, false, TabRows>(lds, io);
+  for (int i = threadIdx.x; i < N; i += NT) tl[i] = stab[i];
+  __syncthreads();
+  io.stab = tl;
+  mid_and_last<T, FWD, N, B, NT, R0, 0, false, TabRows, Rs...>(lds, tw, io);
+}
+
 // Spectral solve along the column: forward FFT, per-mode multiply by
 // -scale/|k|^2 (k=0 -> 0), inverse FFT, all in one LDS tile.  Column i0 + b
 // of the tile is kx + nxh * ky_index; the row is the kz index.
