@@ -391,6 +391,42 @@ bool forward_many_dev(State& S, const real_t* in, long long dim_in, cplx_t* out,
   const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
   const long long nxy = (long long)nxhpc * nyc;
   const long long nreal = (long long)nx * ny * nz;
+  const long long nspec = nxy * nzc;
+  // no cuts, packed variables, no overlap: X r2c of every variable straight
+  // into the output, then Y and Z in place there (as the single-variable path)
+  const char* ib = (const char*)in;
+  const char* ob = (const char*)out;
+  const bool alias = ob < ib + (size_t)nreal * nv * sizeof(real_t) && ib < ob + (size_t)nspec * nv * sizeof(cplx_t);
+  if (g_x_into_out && !S.stride1 && ny == nyc && nz == nzc && dim_in == nreal && dim_out == nspec && !alias) {
+    tstart(S, 5, s);
+    launch_x_r2c(in, out, (long long)ny * nz * nv, nx, nxhpc, nx, nxhpc, s);
+    tstop(S, 5, s);
+    ColSpec y{};
+    y.in = out; y.out = out;
+    y.si_in = 1; y.sl_in = nxhpc; y.so_in = nxy;
+    y.si_out = 1; y.sl_out = nxhpc; y.so_out = nxy;
+    y.n_inner = nxhpc; y.n_outer = nz * nv; y.n = ny; y.kind = COL_C2C; y.fwd = true;
+    y.nld = ny; y.ld_lo = ny; y.ld_hi = ny; y.ld_shift = 0;
+    y.nst = ny; y.st_lo = ny; y.st_shift = 0;
+    y.rev = tile_order_alternate();
+    tstart(S, 7, s);
+    launch_col(y, s);
+    tstop(S, 7, s);
+    tstart(S, 8, s);
+    if (!none) {
+      ColSpec z{};
+      z.in = out; z.out = out;
+      z.n_inner = (int)nxy; z.n_outer = nv;
+      z.si_in = 1; z.sl_in = nxy; z.so_in = nspec;
+      z.si_out = 1; z.sl_out = nxy; z.so_out = nspec;
+      z.n = nz; z.kind = kind; z.fwd = true;
+      z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
+      z.nst = nz; z.st_lo = nz; z.st_shift = 0;
+      launch_col(z, s);
+    }
+    tstop(S, 8, s);
+    return true;
+  }
   // X r2c of every variable into w1 [nv][nz][ny][nxhpc]
   tstart(S, 5, s);
   if (dim_in == nreal) {
@@ -461,6 +497,7 @@ bool backward_many_dev(State& S, cplx_t* in, long long dim_in, real_t* out, long
       z.si_in = 1; z.sl_in = nxy; z.so_in = dim_in;
       z.si_out = 1; z.sl_out = nxy; z.so_out = dim_in;
       z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
+      z.rev = tile_order_alternate();
       launch_col(z, s);
     }
   } else {
@@ -507,7 +544,7 @@ bool backward_many_dev(State& S, cplx_t* in, long long dim_in, real_t* out, long
   // X c2r of every variable
   tstart(S, 12, s);
   if (dim_out == (long long)nx * ny * nz) {
-    launch_x_c2r(S.w1, out, (long long)ny * nz * nv, nxhpc, nx, nx, nxhpc, s);
+    launch_x_c2r(S.w1, out, (long long)ny * nz * nv, nxhpc, nx, nx, nxhpc, s, tile_order_alternate());
   } else {
     for (int j = 0; j < nv; j++)
       launch_x_c2r(S.w1 + (size_t)j * nxhpc * ny * nz, out + (size_t)j * dim_out, (long long)ny * nz, nxhpc, nx,

@@ -688,12 +688,6 @@ __global__ void k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, 
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) out[i] = in[i];
 }
 
-template <class T>
-__global__ void k_scale(cpx<T>* p, long long n, double s) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i < n) { cpx<T> v = p[i]; p[i] = mk<T>((T)(v.x * s), (T)(v.y * s)); }
-}
-
 // ---------------------------------------------------------------------------
 // host side: length plans
 // ---------------------------------------------------------------------------
@@ -1335,12 +1329,6 @@ void launch_copy16(const void* in, void* out, long long n16, hipStream_t st) {
   const long long blocks = std::min<long long>((n16 + 255) / 256, 256LL * 64);
   k_copy16<<<(unsigned)blocks, 256, 0, st>>>((const uint4*)in, (uint4*)out, n16);
   check_hip(hipGetLastError(), "k_copy16 launch");
-}
-
-void launch_scale(cplx_t* p, long long n, double s, hipStream_t st) {
-  if (n <= 0) return;
-  k_scale<real_t><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, s);
-  check_hip(hipGetLastError(), "k_scale launch");
 }
 
 }  // namespace p3amd

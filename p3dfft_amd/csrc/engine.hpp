@@ -112,7 +112,4 @@ void set_specialised(bool on);
 // out[i] = in[i] over n16 16-byte vectors
 void launch_copy16(const void* in, void* out, long long n16, hipStream_t st);
 
-// out[i] *= s over n complex elements (used by callers that normalise)
-void launch_scale(cplx_t* p, long long n, double s, hipStream_t st);
-
 }  // namespace p3amd
