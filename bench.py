@@ -283,6 +283,35 @@ def serve_leg(n_req, slots, load_frac=0.7):
             "mean_points_per_request": float(np.mean([np.prod(s) for s in shapes]))}
 
 
+def batch_leg(n_req=256, slots=8, reps=20):
+    """A fixed batch of small mixed-size solves (axes from {32,...,128}, FP32)
+    replayed `reps` times: issued eagerly on `slots` streams, and captured once
+    into a HIP graph of `slots` branches (p3dfft_poisson_graph_*)."""
+    from p3dfft_amd import serve
+    shapes = serve.make_requests(n_req, seed=2002, sizes=(32, 48, 64, 96, 128))
+    S = serve.PoissonServer(shapes, precision=4, slots=slots)
+    G = None
+    try:
+        S.arena.close()
+        S.arena = serve.Arena(S.lib, sum((w + 255) & ~255 for w in S.work))
+        idx = list(range(n_req))
+        S.fill()
+        S.run_streams(idx, reps=2)  # warm
+        t_eager = min(S.run_streams(idx, reps=reps) for _ in range(3))
+        G = S.graph(idx, nstreams=slots)
+        G.run(reps=2)  # warm
+        t_graph = min(G.run(reps=reps) for _ in range(3))
+    finally:
+        if G is not None:
+            G.close()
+        S.close()
+    eager, graph = n_req * reps / t_eager, n_req * reps / t_graph
+    return {"requests": n_req, "branches": slots, "reps": reps,
+            "eager_streams_solves_per_s": eager, "graph_solves_per_s": graph,
+            "speedup_graph_vs_eager": graph / eager,
+            "mean_points_per_request": float(np.mean([np.prod(s) for s in shapes]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,6 +405,10 @@ def main():
         line["serve"] = serve_leg(args.serve_requests, args.serve_slots)
         log("serve %.1f solves/s (one-at-a-time %.1f)" % (line["serve"]["solves_per_s"],
                                                           line["serve"]["one_at_a_time_solves_per_s"]))
+        log("batched leg (graph-captured vs eager streams)")
+        line["batched"] = batch_leg(slots=args.serve_slots)
+        log("batched %.1f solves/s graph, %.1f eager" % (line["batched"]["graph_solves_per_s"],
+                                                          line["batched"]["eager_streams_solves_per_s"]))
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % args.cpu_threads)
         line["cpu_baseline"] = cpu_baseline(n, f, args.cpu_threads)

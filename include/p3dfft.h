@@ -157,6 +157,16 @@ void p3dfft_poisson_set_fused(int on);
 /* The multiply step alone, on a spectrum (nx/2+1, ny, nz) from
  * p3dfft_ftran_r2c_dev: times -1/(|k|^2 nx ny nz), k = 0 set to 0. */
 int p3dfft_poisson_multiply(p3dfft_real *spec, int nx, int ny, int nz, void *stream);
+/* A fixed batch of n solves (shapes: 3n ints nx,ny,nz; fields[i], works[i],
+ * work_bytes[i] as for p3dfft_poisson_solve) captured once into a HIP graph,
+ * spread over nstreams concurrent branches.  Each launch replays every solve
+ * in place on its field, asynchronously on `stream`.  Plans, twiddle tables
+ * and kernel attributes are set up before capture, so the graph holds kernel
+ * launches and branch joins only. */
+int p3dfft_poisson_graph_create(int n, const int *shapes, p3dfft_real *const *fields, void *const *works,
+                                const size_t *work_bytes, int nstreams, void **graph);
+int p3dfft_poisson_graph_launch(void *graph, void *stream);
+int p3dfft_poisson_graph_destroy(void *graph);
 /* build the plan and twiddle table of a transform length ahead of time */
 int p3dfft_amd_prepare_length(int n);
 /* i.i.d. N(0,1) values from a counter-based generator keyed by seed */

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "engine.hpp"
 #include "p3dfft_amd_internal.hpp"
@@ -158,6 +159,132 @@ int p3dfft_poisson_multiply(p3amd::real_t* spec, int nx, int ny, int nz, void* s
   k_poisson<real_t><<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(
       (cplx_t*)spec, nxhp, ny, nz, 1.0 / ((double)nx * ny * nz));
   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// A fixed batch of solves captured once into a HIP graph and replayed: the
+// launches of every solve (five to seven kernels each) are issued by one
+// graph launch.  The solves are spread over `nstreams` forked branches
+// (largest first, each to the branch with the least work so far), so
+// independent solves overlap on the device as they do on separate streams.
+struct PoissonGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int nsolves = 0;
+};
+
+int p3dfft_poisson_graph_create(int n, const int* shapes, p3amd::real_t* const* fields, void* const* works,
+                                const size_t* work_bytes, int nstreams, void** graph_out) {
+  if (!graph_out || n <= 0 || !shapes || !fields || !works || !work_bytes) {
+    set_error("p3dfft_poisson_graph_create: invalid arguments");
+    return -1;
+  }
+  *graph_out = nullptr;
+  for (int i = 0; i < n; i++) {
+    const int nx = shapes[3 * i], ny = shapes[3 * i + 1], nz = shapes[3 * i + 2];
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > max_len() || ny > max_len() || nz > max_len()) {
+      set_error("p3dfft_poisson_graph_create: invalid grid");
+      return -1;
+    }
+    if (!fields[i] || !works[i] || work_bytes[i] < poisson_work_bytes(nx, ny, nz)) {
+      set_error("p3dfft_poisson_graph_create: missing field or work area too small");
+      return -1;
+    }
+  }
+  const int nb = std::max(1, std::min(nstreams, n));
+  // plans, twiddle tables and kernel attributes first: capture records launches only
+  for (int i = 0; i < 3 * n; i++)
+    if (p3dfft_amd_prepare_length(shapes[i]) != 0) return -1;
+  // largest-first assignment to the least-loaded branch
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  auto pts = [&](int i) { return (double)shapes[3 * i] * shapes[3 * i + 1] * shapes[3 * i + 2]; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pts(a) > pts(b); });
+  std::vector<double> load(nb, 0.0);
+  std::vector<std::vector<int>> branch(nb);
+  for (int i : order) {
+    const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    branch[b].push_back(i);
+    load[b] += pts(i);
+  }
+  hipStream_t cap = nullptr;
+  std::vector<hipStream_t> side(nb, nullptr);
+  std::vector<hipEvent_t> done(nb, nullptr);
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool capturing = false;
+  int rc = -1;
+  try {
+    check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+    check_hip(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+    for (int b = 0; b < nb; b++) {
+      check_hip(hipStreamCreateWithFlags(&side[b], hipStreamNonBlocking), "hipStreamCreate");
+      check_hip(hipEventCreateWithFlags(&done[b], hipEventDisableTiming), "hipEventCreate");
+    }
+    check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    capturing = true;
+    check_hip(hipEventRecord(fork, cap), "hipEventRecord");
+    for (int b = 0; b < nb; b++) {
+      check_hip(hipStreamWaitEvent(side[b], fork, 0), "hipStreamWaitEvent");
+      for (int i : branch[b])
+        if (p3dfft_poisson_solve(shapes[3 * i], shapes[3 * i + 1], shapes[3 * i + 2], fields[i], works[i],
+                                 work_bytes[i], (void*)side[b]) != 0)
+          throw Error{std::string(), -1};  // the solve set the message
+      check_hip(hipEventRecord(done[b], side[b]), "hipEventRecord");
+      check_hip(hipStreamWaitEvent(cap, done[b], 0), "hipStreamWaitEvent");
+    }
+    capturing = false;
+    check_hip(hipStreamEndCapture(cap, &graph), "hipStreamEndCapture");
+    check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    PoissonGraph* g = new PoissonGraph;
+    g->graph = graph;
+    g->exec = exec;
+    g->nsolves = n;
+    *graph_out = g;
+    graph = nullptr;
+    exec = nullptr;
+    rc = 0;
+  } catch (const Error& e) {
+    if (!e.msg.empty()) set_error(e.msg);
+  }
+  if (capturing) {
+    hipGraph_t partial = nullptr;
+    (void)hipStreamEndCapture(cap, &partial);
+    if (partial) (void)hipGraphDestroy(partial);
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (int b = 0; b < nb; b++) {
+    if (done[b]) (void)hipEventDestroy(done[b]);
+    if (side[b]) (void)hipStreamDestroy(side[b]);
+  }
+  if (fork) (void)hipEventDestroy(fork);
+  if (cap) (void)hipStreamDestroy(cap);
+  return rc;
+}
+
+// Replay every captured solve on `stream` (asynchronous; the fields are
+// solved in place again, so the caller refreshes them between replays).
+int p3dfft_poisson_graph_launch(void* graph, void* stream) {
+  if (!graph) {
+    set_error("p3dfft_poisson_graph_launch: null graph");
+    return -1;
+  }
+  const hipError_t e = hipGraphLaunch(((PoissonGraph*)graph)->exec, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int p3dfft_poisson_graph_destroy(void* graph) {
+  if (!graph) return 0;
+  PoissonGraph* g = (PoissonGraph*)graph;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return 0;
 }
 
 // Fill n values with i.i.d. N(0,1) from a counter-based generator keyed by seed.

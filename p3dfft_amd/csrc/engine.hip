@@ -723,6 +723,7 @@ static bool is_templated(int r) {
 
 static std::mutex g_len_mu;
 static std::map<int, Len1D> g_lens;
+static void prep_kernels(const Len1D& L);
 
 const Len1D& get_len(int n) {
   std::lock_guard<std::mutex> lk(g_len_mu);
@@ -765,6 +766,7 @@ const Len1D& get_len(int n) {
     check_hip(hipDeviceSynchronize(), "k_gen_tw");
     check_hip(hipFree(didx), "hipFree");
   }
+  prep_kernels(L);
   auto res = g_lens.emplace(n, L);
   return res.first->second;
 }
@@ -827,27 +829,27 @@ constexpr int kEPTc = 16;
 struct GoXr2c {
   template <int NT>
   static void go(const XArgs& a, unsigned g, size_t b, hipStream_t s) {
-    static bool once = (allow_big_lds(k_x_r2c<real_t, NT, kEPTc>), true);
-    (void)once;
     k_x_r2c<real_t, NT, kEPTc><<<g, NT, b, s>>>(a);
   }
+  template <int NT>
+  static void prep() { allow_big_lds(k_x_r2c<real_t, NT, kEPTc>); }
 };
 struct GoXc2r {
   template <int NT>
   static void go(const XArgs& a, unsigned g, size_t b, hipStream_t s) {
-    static bool once = (allow_big_lds(k_x_c2r<real_t, NT, kEPTc>), true);
-    (void)once;
     k_x_c2r<real_t, NT, kEPTc><<<g, NT, b, s>>>(a);
   }
+  template <int NT>
+  static void prep() { allow_big_lds(k_x_c2r<real_t, NT, kEPTc>); }
 };
 template <bool FWD>
 struct GoCol {
   template <int NT>
   static void go(const ColArgs& a, unsigned g, size_t b, hipStream_t s) {
-    static bool once = (allow_big_lds(k_col<real_t, FWD, NT, kEPTc>), true);
-    (void)once;
     k_col<real_t, FWD, NT, kEPTc><<<g, NT, b, s>>>(a);
   }
+  template <int NT>
+  static void prep() { allow_big_lds(k_col<real_t, FWD, NT, kEPTc>); }
 };
 
 template <class F, class A>
@@ -922,6 +924,7 @@ struct CtInfo {
   void (*xr2c)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*xc2r)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*zsolve)(const ColArgs&, double, int, int, unsigned, size_t, hipStream_t);
+  void (*prep)();
 };
 
 template <int... Rs>
@@ -942,14 +945,10 @@ struct CtLen {
   static constexpr int WX = ct_waves(DX + (TWLX ? TWB : 0), NTX);
   template <class K>
   static void go(K k, unsigned g, int nt, size_t bytes, hipStream_t s, const ColArgs& a) {
-    static bool once = (allow_big_lds(k), true);
-    (void)once;
     k<<<g, nt, bytes, s>>>(a);
   }
   template <class K>
   static void gox(K k, unsigned g, int nt, size_t bytes, hipStream_t s, const XArgs& a) {
-    static bool once = (allow_big_lds(k), true);
-    (void)once;
     k<<<g, nt, bytes, s>>>(a);
   }
   static void col(bool fwd, const ColArgs& a, unsigned g, size_t bytes, hipStream_t s) {
@@ -985,10 +984,27 @@ struct CtLen {
     else gox(k_x_c2r_ct<real_t, BX, NTX, RL, WX, TWLX, false>, g, NTX, bytes, s, a);
   }
   static void zsolve(const ColArgs& a, double scale, int nxh, int ny, unsigned g, size_t bytes, hipStream_t s) {
-    auto k = k_zsolve_ct<real_t, TX, NTC, RL, WC, TWLC>;
-    static bool once = (allow_big_lds(k), true);
-    (void)once;
-    k<<<g, NTC, bytes, s>>>(a, scale, nxh, ny);
+    k_zsolve_ct<real_t, TX, NTC, RL, WC, TWLC><<<g, NTC, bytes, s>>>(a, scale, nxh, ny);
+  }
+  // raise the dynamic-LDS limit of every kernel of this length (host-side
+  // attribute calls, made once when the length is planned, so that launches
+  // alone remain for stream capture)
+  template <bool FWD>
+  static void prep_dir() {
+    allow_big_lds(k_col_ct<real_t, FWD, TX, NTC, RL, WC, TWLC>);
+    allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, TabRows>);
+    allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, true, unsigned>);
+    allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, unsigned long long>);
+    allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, unsigned>);
+  }
+  static void prep() {
+    prep_dir<true>();
+    prep_dir<false>();
+    allow_big_lds(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX, true>);
+    allow_big_lds(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX, false>);
+    allow_big_lds(k_x_c2r_ct<real_t, BX, NTX, RL, WX, TWLX, true>);
+    allow_big_lds(k_x_c2r_ct<real_t, BX, NTX, RL, WX, TWLX, false>);
+    allow_big_lds(k_zsolve_ct<real_t, TX, NTC, RL, WC, TWLC>);
   }
   static CtInfo info() {
     CtInfo c;
@@ -998,7 +1014,7 @@ struct CtLen {
     c.nr = (int)sizeof...(Rs);
     for (int i = 0; i < c.nr; i++) c.radix[i] = rs[i];
     c.twl_col = TWLC; c.twl_x = TWLX; c.twb = TWB;
-    c.col = &col; c.colio = &colio; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve;
+    c.col = &col; c.colio = &colio; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve; c.prep = &prep;
     return c;
   }
 };
@@ -1019,6 +1035,30 @@ static const CtInfo* find_ct(const Len1D& L) {
   for (int i = 0; i < c.nr; i++)
     if (c.radix[i] != L.d.radix[i]) return nullptr;
   return &c;
+}
+
+template <class F>
+static void prep_generic_nt() {
+  F::template prep<64>();
+  F::template prep<128>();
+  F::template prep<256>();
+  F::template prep<512>();
+  F::template prep<1024>();
+}
+
+// Kernel attributes (the dynamic-LDS limit) are set when a length is first
+// planned, never at launch: launches are then the only device-side work of a
+// transform, which is what stream capture into a graph requires.
+static void prep_kernels(const Len1D& L) {
+  static bool generic_done = false;
+  if (!generic_done) {
+    prep_generic_nt<GoXr2c>();
+    prep_generic_nt<GoXc2r>();
+    prep_generic_nt<GoCol<true>>();
+    prep_generic_nt<GoCol<false>>();
+    generic_done = true;
+  }
+  if (const CtInfo* c = find_ct(L)) c->prep();
 }
 
 static bool g_use_ct = true;

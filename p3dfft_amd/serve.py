@@ -40,6 +40,12 @@ def _bind(L):
     L.p3dfft_poisson_multiply.restype = ctypes.c_int
     L.p3dfft_amd_prepare_length.argtypes = [ctypes.c_int]
     L.p3dfft_amd_prepare_length.restype = ctypes.c_int
+    L.p3dfft_poisson_graph_create.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.p3dfft_poisson_graph_create.restype = ctypes.c_int
+    L.p3dfft_poisson_graph_launch.argtypes = [vp, vp]
+    L.p3dfft_poisson_graph_launch.restype = ctypes.c_int
+    L.p3dfft_poisson_graph_destroy.argtypes = [vp]
+    L.p3dfft_poisson_graph_destroy.restype = ctypes.c_int
     L._serve_bound = True
     return L
 
@@ -247,6 +253,33 @@ class PoissonServer:
         finally:
             L.p3dfft_amd_free(spec)
 
+    def graph(self, idx, nstreams=8):
+        """A PoissonGraph of the solves of requests `idx` on their fields."""
+        return PoissonGraph(self, idx, nstreams)
+
+    def run_streams(self, idx, reps=1):
+        """The same batch issued eagerly: request k of `idx` on stream k mod
+        slots, each with its own work area, `reps` times.  Returns seconds."""
+        L = self.lib
+        blks = [self.arena.alloc(self.work[i]) for i in idx]
+        try:
+            if any(b is None for b in blks):
+                raise P3DFFTError("work budget below the batch's work areas")
+            L.p3dfft_amd_synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                for k, i in enumerate(idx):
+                    rc = L.p3dfft_poisson_solve(*self.shapes[i], self.field_ptr(i), self.arena.ptr(blks[k]),
+                                                self.work[i], self.streams[k % self.slots])
+                    if rc:
+                        raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+            L.p3dfft_amd_synchronize()
+            return time.perf_counter() - t0
+        finally:
+            for b in blks:
+                if b is not None:
+                    self.arena.release(b)
+
     def close(self):
         L = self.lib
         L.p3dfft_amd_synchronize()
@@ -259,3 +292,56 @@ class PoissonServer:
             L.p3dfft_amd_free(self.fields)
             self.fields = None
 
+
+
+class PoissonGraph:
+    """A fixed batch of solves (requests `idx` of a PoissonServer, in place on
+    their resident fields) captured once into a HIP graph by
+    p3dfft_poisson_graph_create and replayed by launch(): one host call issues
+    every kernel of the batch.  Work areas (one per solve) are held for the
+    graph's lifetime."""
+
+    def __init__(self, server, idx, nstreams=8):
+        L = server.lib
+        self.lib = L
+        self.idx = list(idx)
+        n = len(self.idx)
+        works = [server.work[i] for i in self.idx]
+        offs = np.concatenate([[0], np.cumsum([(w + 255) & ~255 for w in works])]).astype(np.int64)
+        self.work_base = L.p3dfft_amd_malloc(int(offs[-1]))
+        if not self.work_base:
+            raise P3DFFTError("graph work allocation failed")
+        shapes = (ctypes.c_int * (3 * n))(*[v for i in self.idx for v in server.shapes[i]])
+        fields = (ctypes.c_void_p * n)(*[server.field_ptr(i) for i in self.idx])
+        wptr = (ctypes.c_void_p * n)(*[self.work_base + int(o) for o in offs[:-1]])
+        wbytes = (ctypes.c_size_t * n)(*works)
+        h = ctypes.c_void_p()
+        if L.p3dfft_poisson_graph_create(n, shapes, fields, wptr, wbytes, int(nstreams), ctypes.byref(h)):
+            L.p3dfft_amd_free(self.work_base)
+            self.work_base = None
+            raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+        self.handle = h.value
+
+    def launch(self, stream=None):
+        if self.lib.p3dfft_poisson_graph_launch(self.handle, stream):
+            raise P3DFFTError(self.lib.p3dfft_amd_last_error().decode())
+
+    def run(self, reps=1):
+        """`reps` back-to-back replays; returns seconds."""
+        L = self.lib
+        L.p3dfft_amd_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            self.launch()
+        L.p3dfft_amd_synchronize()
+        return time.perf_counter() - t0
+
+    def close(self):
+        L = self.lib
+        L.p3dfft_amd_synchronize()
+        if self.handle:
+            L.p3dfft_poisson_graph_destroy(self.handle)
+            self.handle = None
+        if self.work_base:
+            L.p3dfft_amd_free(self.work_base)
+            self.work_base = None

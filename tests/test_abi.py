@@ -90,3 +90,25 @@ def test_fortran_module_built():
     assert os.path.exists(os.path.join(lib, "mod", "p3dfft.mod"))
     assert os.path.exists(os.path.join(lib, "libp3dfft_amd_f.so"))
     assert os.path.exists(os.path.join(lib, "mod_sp", "p3dfft.mod"))
+
+
+def test_poisson_graph_rejects_invalid_batches_without_gpu():
+    """Batch validation (shapes, fields, work areas) precedes any device work."""
+    import ctypes
+    from p3dfft_amd import serve
+    L = serve._bind(p3dfft_amd.load(4))
+    h = ctypes.c_void_p()
+    assert L.p3dfft_poisson_graph_create(0, None, None, None, None, 1, ctypes.byref(h)) == -1
+    shapes = (ctypes.c_int * 3)(32, 0, 32)
+    fields = (ctypes.c_void_p * 1)(16)
+    works = (ctypes.c_void_p * 1)(16)
+    wbytes = (ctypes.c_size_t * 1)(1 << 20)
+    assert L.p3dfft_poisson_graph_create(1, shapes, fields, works, wbytes, 1, ctypes.byref(h)) == -1
+    assert b"invalid grid" in L.p3dfft_amd_last_error()
+    shapes = (ctypes.c_int * 3)(32, 32, 32)
+    wbytes = (ctypes.c_size_t * 1)(100)
+    assert L.p3dfft_poisson_graph_create(1, shapes, fields, works, wbytes, 1, ctypes.byref(h)) == -1
+    assert b"work area too small" in L.p3dfft_amd_last_error()
+    assert h.value is None
+    assert L.p3dfft_poisson_graph_destroy(None) == 0
+    L.p3dfft_amd_clear_error()

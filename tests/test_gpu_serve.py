@@ -143,3 +143,66 @@ def test_reference_api_one_at_a_time_matches():
             assert rel(S.download(i), fp64_solve(rhs[i])) < 1e-12, shapes[i]
     finally:
         S.close()
+
+
+@pytest.mark.parametrize("precision,fused", [(4, True), (8, True), (4, False)])
+def test_graph_replays_match_eager(precision, fused):
+    """A batch captured into a HIP graph (specialised, generic and prime
+    lengths; fused Z-solve or the composed multiply pass) gives, on every
+    replay, bit-for-bit the fields of the same solves issued eagerly on
+    streams, with eager fills and solves between the replays."""
+    shapes = [(32, 36, 48), (17, 32, 20), (64, 64, 64), (48, 40, 32), (36, 17, 64), (96, 32, 48), (32, 32, 32)]
+    S = serve.PoissonServer(shapes, precision=precision, slots=3, work_budget=None)
+    S.arena.close()
+    S.arena = serve.Arena(S.lib, sum((w + 255) & ~255 for w in S.work))
+    idx = list(range(len(shapes)))
+    S.lib.p3dfft_poisson_set_fused(1 if fused else 0)
+    G = None
+    try:
+        S.fill()
+        rhs = [S.download(i) for i in idx]
+        S.run_streams(idx, reps=1)
+        once = [S.download(i) for i in idx]
+        S.fill()
+        S.run_streams(idx, reps=2)
+        twice = [S.download(i) for i in idx]
+        G = S.graph(idx, nstreams=3)
+        for rep in range(3):
+            S.fill()
+            S.solve_alone(rep)  # eager work between replays, then a fresh right-hand side
+            S.fill([rep])
+            G.launch()
+            S.lib.p3dfft_amd_synchronize()
+            for i in idx:
+                assert np.array_equal(S.download(i), once[i]), (rep, i, shapes[i])
+        S.fill()
+        G.run(reps=2)
+        for i in idx:
+            assert np.array_equal(S.download(i), twice[i]), (i, shapes[i])
+        tol = 2e-6 if precision == 4 else 1e-12
+        for i in idx:
+            assert rel(once[i], fp64_solve(rhs[i].astype(np.float64))) < tol * 10, (i, shapes[i])
+    finally:
+        S.lib.p3dfft_poisson_set_fused(1)
+        if G is not None:
+            G.close()
+        S.close()
+
+
+def test_graph_create_rejects_small_work_area():
+    import ctypes
+    S = serve.PoissonServer([(32, 32, 32)], precision=4, slots=1)
+    try:
+        L = S.lib
+        shapes = (ctypes.c_int * 3)(32, 32, 32)
+        fields = (ctypes.c_void_p * 1)(S.field_ptr(0))
+        blk = S.arena.alloc(S.work[0])
+        works = (ctypes.c_void_p * 1)(S.arena.ptr(blk))
+        wbytes = (ctypes.c_size_t * 1)(S.work[0] - 1)
+        h = ctypes.c_void_p()
+        assert L.p3dfft_poisson_graph_create(1, shapes, fields, works, wbytes, 2, ctypes.byref(h)) == -1
+        assert b"work area" in L.p3dfft_amd_last_error()
+        assert h.value is None
+        S.arena.release(blk)
+    finally:
+        S.close()
