@@ -210,6 +210,16 @@ void p3dfft_amd_set_wide_offsets(int on);
  * (the Z pass then reads lines the Y pass left in the last-level cache);
  * 0: X r2c into the work buffer, Y from there into the output. */
 void p3dfft_amd_set_x_into_out(int on);
+/* 1: from the next p3dfft_setup on, the X and Y passes of a direction run as
+ * one persistent launch (a work queue per XCD; each z-plane's intermediate is
+ * re-read from that XCD's L2) where the grid allows (nx == ny specialised
+ * lengths 128..512, unpruned X/Y, standard layout, output not overlapping the
+ * input); 0 (default): separate X and Y kernels, which measured 2-3x faster
+ * on MI355X at 256^3..512^3. */
+void p3dfft_amd_set_fused_xy(int on);
+/* -1: the current setup has no fused passes; else the error word of the last
+ * fused launch (0 = none, 1 = a dependency wait timed out).  Synchronises. */
+int p3dfft_amd_fused_xy_status(void);
 /* sizeof(p3dfft_real) of this build: 8 or 4. */
 int p3dfft_amd_precision(void);
 

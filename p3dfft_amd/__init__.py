@@ -88,6 +88,8 @@ def load(precision=8):
     L.p3dfft_amd_set_tile_order.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_wide_offsets.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_x_into_out.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_set_fused_xy.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_fused_xy_status.restype = ctypes.c_int
     L.p3dfft_share_set_fused.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_chunk_bytes.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_malloc.argtypes = [ctypes.c_size_t]

@@ -224,7 +224,12 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
   // there, so the Y pass reads (and leaves in the last-level cache) the lines
   // the Z pass reads next
   const bool xin = g_x_into_out && direct && dny == 0 && !alias && zc == nz;
-  if (xin) {
+  if (xin && S.fuse && fuse_xy_enabled()) {
+    // X and Y in one persistent launch, each plane's X output re-read from L2
+    tstart(S, 5, s);
+    launch_xy_fwd(S.fuse, in, out, s);
+    tstop(S, 5, s);
+  } else if (xin) {
     tstart(S, 5, s);
     launch_x_r2c(in, out, (long long)ny * nz, nx, nxhpc, nx, nxhpc, s);
     tstop(S, 5, s);
@@ -329,6 +334,20 @@ bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hi
     zdst = S.w2;
   }
   tstop(S, 9, s);
+  // unpruned X/Y, output not overlapping the Z result: Y and X c2r in one
+  // persistent launch, each plane's Y output re-read from L2 (ring in w1)
+  if (S.fuse && fuse_xy_enabled() && !S.stride1 && dny == 0 && nxhpc == nx / 2 + 1 &&
+      fuse_ring_elems(S.fuse) <= (long long)nxhpc * ny * nz) {
+    const char* zb = (const char*)zdst;
+    const char* ob = (const char*)out;
+    const bool alias = ob < zb + (size_t)nxy * nz * sizeof(cplx_t) && zb < ob + (size_t)nx * ny * nz * sizeof(real_t);
+    if (!alias) {
+      tstart(S, 12, s);
+      launch_yx_bwd(S.fuse, zdst, out, S.w1, s);
+      tstop(S, 12, s);
+      return true;
+    }
+  }
   // Y c2c backward then X c2r, chunk by chunk over z-planes (the chunk's
   // intermediate stays in the last-level cache); seg_copy_y + seg_zero_y and
   // seg_copy_x + seg_zero_x (x > nxhpc) are folded into the loads
@@ -610,6 +629,8 @@ void release_state() {
   g.evpool.clear();
   if (g.w1) (void)hipFree(g.w1);
   if (g.w2) (void)hipFree(g.w2);
+  fuse_plan_free(g.fuse);
+  g.fuse = nullptr;
   if (g.stage_a) (void)hipFree(g.stage_a);
   if (g.stage_b) (void)hipFree(g.stage_b);
   g.w1 = g.w2 = nullptr;
@@ -677,6 +698,8 @@ void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int
   get_len(g.nx);
   get_len(g.ny);
   get_len(g.nz);
+  if (fuse_xy_enabled() && !g.stride1 && g.nxhpc == g.nx / 2 + 1 && g.nyc == g.ny)
+    g.fuse = fuse_plan_create(g.nx, g.ny, g.nz);
   g.set = true;
   memsize[0] = g.memsize[0];
   memsize[1] = g.memsize[1];
@@ -933,6 +956,23 @@ void p3dfft_amd_set_specialised(int on) { set_specialised(on != 0); }
 void p3dfft_amd_set_tile_order(int alternate) { set_tile_order(alternate != 0); }
 void p3dfft_amd_set_wide_offsets(int on) { set_wide_offsets(on != 0); }
 void p3dfft_amd_set_x_into_out(int on) { g_x_into_out = (on != 0); }
+
+// 1: fused X/Y passes for the next setups, where the shape has them;
+// 0 (default): separate passes.
+void p3dfft_amd_set_fused_xy(int on) { set_fuse_xy(on != 0); }
+
+// -1: the current setup has no fused X/Y passes; otherwise the error word of
+// the last fused launch (0 = none; 1 = a dependency wait timed out).
+// Synchronises the device.
+int p3dfft_amd_fused_xy_status(void) {
+  if (!g.set || !g.fuse) return -1;
+  try {
+    return (int)fuse_status(g.fuse);
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return -2;
+  }
+}
 
 // Device memory helpers for callers without their own HIP runtime bindings.
 void* p3dfft_amd_malloc(size_t bytes) {

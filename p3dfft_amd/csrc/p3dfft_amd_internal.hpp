@@ -31,6 +31,8 @@ struct State {
   cplx_t* w1 = nullptr;
   cplx_t* w2 = nullptr;
   int nv_alloc = 0;  // variables the work buffers hold (grown by the _many calls)
+  // fused X/Y passes of this grid (null when the shape has none)
+  FusePlan* fuse = nullptr;
   // share mode (one rank of a P1 x P2 grid): Z pencil work buffer and splits
   bool share = false;
   cplx_t* w3 = nullptr;

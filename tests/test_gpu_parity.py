@@ -489,3 +489,81 @@ def test_x_into_out_bit_identical(precision, shape, op):
             P.clean()
             P.lib.p3dfft_amd_set_x_into_out(1)
     assert np.array_equal(res[0], res[1])
+
+
+FUSED_SHAPES = [(256, 256, 64), (384, 384, 30), (128, 128, 33), (160, 160, 10), (512, 512, 20), (192, 192, 7)]
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+@pytest.mark.parametrize("shape", FUSED_SHAPES)
+def test_fused_xy_matches_separate_passes_and_oracle(precision, shape):
+    """The fused X/Y launches (opt-in; a work queue per XCD; each plane's X output,
+    forward, or Y output, backward, re-read from L2) give the spectrum and the
+    round trip of the separate X and Y kernels within a tenth of the
+    precision's tolerance, match the oracle, and record no error.  Shapes
+    cover every fused length class, odd plane counts and a plane count below
+    the backward ring (forward fused only)."""
+    nx, ny, nz = shape
+    f = np.random.default_rng(41).standard_normal((nz, ny, nx))
+    res = []
+    for on in (1, 0):
+        P = p3dfft_amd.P3DFFT(precision)
+        P.lib.p3dfft_amd_set_fused_xy(on)
+        P.setup((1, 1), nx, ny, nz)
+        try:
+            s = P.ftran_r2c(f, "fft")
+            b = P.btran_c2r(s, "tff")
+            st = P.lib.p3dfft_amd_fused_xy_status()
+            res.append((s, b, st))
+        finally:
+            P.clean()
+            P.lib.p3dfft_amd_set_fused_xy(0)
+    (s1, b1, st1), (s0, b0, _) = res
+    esize = 8 if precision == 4 else 16
+    if (nx // 2 + 1) * nx * esize > 1200 * 1024:
+        # no fused kernel: the plane does not fit the L2 share; same passes run
+        assert st1 == -1
+        assert np.array_equal(s1, s0) and np.array_equal(b1, b0)
+        return
+    assert st1 == 0, "fused launch reported error word %d" % st1
+    tol = TOL[precision]
+    assert rel(s1, s0) < tol / 10
+    assert rel(b1, b0) < tol / 10
+    if nx * ny * nz <= 256 * 256 * 64:
+        ref = oplan(nx, ny, nz).ftran_r2c(f)
+        assert rel(s1, ref) < tol
+    assert np.abs(b1 / (nx * ny * nz) - f).max() < PREC[precision] * nx * ny * nz * 0.25
+
+
+def test_fused_xy_in_place_backward_uses_separate_passes():
+    """btran with the output over its input (in-place use) keeps the separate
+    passes, whose plane order is safe for the overlap; results match the
+    out-of-place call."""
+    nx = ny = 256
+    nz = 16
+    f = np.random.default_rng(43).standard_normal((nz, ny, nx))
+    p3dfft_amd.load(4).p3dfft_amd_set_fused_xy(1)
+    try:
+        _in_place_backward(nx, ny, nz, f)
+    finally:
+        p3dfft_amd.load(4).p3dfft_amd_set_fused_xy(0)
+
+
+def _in_place_backward(nx, ny, nz, f):
+    with lib_setup(4, nx, ny, nz) as P:
+        s = P.ftran_r2c(f, "fft")
+        out_of_place = P.btran_c2r(s.copy(), "tff")
+        L = P.lib
+        nbytes = (nx // 2 + 1) * ny * nz * 8
+        buf = L.p3dfft_amd_malloc(nbytes)
+        try:
+            sc = np.ascontiguousarray(s.astype(np.complex64))
+            L.p3dfft_amd_memcpy_h2d(buf, sc.ctypes.data, nbytes)
+            assert L.p3dfft_btran_c2r_dev(buf, buf, b"tff", None) == 0
+            assert L.p3dfft_amd_synchronize() == 0
+            back = np.empty((nz, ny, nx), np.float32)
+            L.p3dfft_amd_memcpy_d2h(back.ctypes.data, buf, back.nbytes)
+        finally:
+            L.p3dfft_amd_free(buf)
+        assert L.p3dfft_amd_fused_xy_status() == 0
+    assert rel(back, out_of_place) < TOL[4] / 10
