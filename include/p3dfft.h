@@ -216,6 +216,18 @@ void p3dfft_amd_set_x_into_out(int on);
  * lengths 128..512, unpruned X/Y, standard layout, output not overlapping the
  * input); 0 (default): separate X and Y kernels, which measured 2-3x faster
  * on MI355X at 256^3..512^3. */
+/* Plan mode for the next p3dfft_setup (counterpart of the reference's
+ * FFTW_MEASURE planning, build/fft_spec.F90:101-108).  0 (default): the
+ * default plan of every length.  1: the setup times candidate plans of each
+ * transform length on its own grid (specialised kernels; generic kernels with
+ * 32/64/128 KiB tiles and 64/128/256-byte column segments; other radix
+ * orders) and the schedule switches, and keeps the fastest.  Measured plans
+ * persist for the process; results stay within each precision's tolerance. */
+void p3dfft_amd_set_plan_mode(int mode);
+/* the last measurement's choices as text; returns the full length */
+int p3dfft_amd_plan_report(char *buf, int len);
+/* default plans again (no active setup and no live Poisson graph) */
+int p3dfft_amd_reset_plans(void);
 void p3dfft_amd_set_fused_xy(int on);
 /* -1: the current setup has no fused passes; else the error word of the last
  * fused launch (0 = none, 1 = a dependency wait timed out).  Synchronises. */

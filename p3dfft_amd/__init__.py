@@ -89,6 +89,10 @@ def load(precision=8):
     L.p3dfft_amd_set_wide_offsets.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_x_into_out.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_fused_xy.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_set_plan_mode.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_plan_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.p3dfft_amd_plan_report.restype = ctypes.c_int
+    L.p3dfft_amd_reset_plans.restype = ctypes.c_int
     L.p3dfft_amd_fused_xy_status.restype = ctypes.c_int
     L.p3dfft_share_set_fused.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_chunk_bytes.argtypes = [ctypes.c_size_t]

@@ -621,6 +621,114 @@ static bool require_full() {
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// plan-time measurement (the counterpart of the reference's FFTW_MEASURE
+// planning, build/fft_spec.F90:101-108): at setup, time candidate plans of
+// every transform length on the setup's own grid and keep the fastest
+// ---------------------------------------------------------------------------
+static int g_plan_mode = 0;  // 0: default plans; 1: measure at setup
+static std::string g_plan_report;
+
+}  // namespace p3amd
+extern "C" int p3dfft_amd_fill_normal(p3amd::real_t* p, long long n, unsigned long long seed, void* stream);
+namespace p3amd {
+
+// ms per forward+backward pair on scratch arrays (one untimed pair first)
+static double time_pair(State& S, const real_t* rin, cplx_t* cspec, real_t* rout, hipStream_t s) {
+  static const unsigned char fop[4] = {'f', 'f', 't', 0}, bop[4] = {'t', 'f', 'f', 0};
+  const bool tm = S.timing;
+  S.timing = false;
+  hipEvent_t a, b;
+  check_hip(hipEventCreate(&a), "hipEventCreate");
+  check_hip(hipEventCreate(&b), "hipEventCreate");
+  auto run = [&](int reps) {
+    check_hip(hipEventRecord(a, s), "hipEventRecord");
+    for (int i = 0; i < reps; i++) {
+      forward_dev(S, rin, cspec, fop, s);
+      backward_dev(S, cspec, rout, bop, s);
+    }
+    check_hip(hipEventRecord(b, s), "hipEventRecord");
+    check_hip(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0;
+    check_hip(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    return (double)ms / reps;
+  };
+  const double t1 = run(1);
+  const int reps = std::max(2, std::min(50, (int)(5.0 / std::max(t1, 1e-3))));
+  const double t = run(reps);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  S.timing = tm;
+  return t;
+}
+
+static void measure_plans(State& S) {
+  const size_t nr = (size_t)S.nx * S.ny * S.nz;
+  const size_t nc = (size_t)S.nxhpc * S.nyc * S.nzc;
+  real_t *rin = nullptr, *rout = nullptr;
+  cplx_t* cspec = nullptr;
+  hipStream_t s = S.stream;
+  check_hip(hipMalloc(&rin, nr * sizeof(real_t)), "hipMalloc(measure)");
+  check_hip(hipMalloc(&rout, nr * sizeof(real_t)), "hipMalloc(measure)");
+  check_hip(hipMalloc(&cspec, nc * sizeof(cplx_t)), "hipMalloc(measure)");
+  (void)p3dfft_amd_fill_normal(rin, (long long)nr, 12345, (void*)s);
+  std::vector<int> lens = {S.nx, S.ny, S.nz};
+  std::sort(lens.begin(), lens.end());
+  lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+  std::string rep;
+  for (int n : lens) {
+    struct Cand {
+      std::vector<int> f;
+      bool ct;
+      int tile, seg;
+    };
+    std::vector<Cand> cands;
+    const std::vector<int> f0 = default_factors(n);
+    if (len_has_ct(n)) cands.push_back({f0, true, 64 * 1024, 128});
+    for (int tile : {32 * 1024, 64 * 1024, 128 * 1024})
+      for (int seg : {64, 128, 256}) cands.push_back({f0, false, tile, seg});
+    std::vector<int> f1(f0.rbegin(), f0.rend());
+    if (f1 != f0) cands.push_back({f1, false, 64 * 1024, 128});
+    std::vector<int> f2 = f0;
+    std::sort(f2.begin(), f2.end());
+    if (f2 != f0 && f2 != f1) cands.push_back({f2, false, 64 * 1024, 128});
+    double best = 1e30;
+    size_t bi = 0;
+    for (size_t i = 0; i < cands.size(); i++) {
+      set_len_plan(n, cands[i].f, cands[i].ct, cands[i].tile, cands[i].seg);
+      try {
+        const double t = time_pair(S, rin, cspec, rout, s);
+        if (t < best) { best = t; bi = i; }
+      } catch (const Error&) {
+        // a candidate whose tiles do not fit is refused before any launch
+        check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+      }
+    }
+    set_len_plan(n, cands[bi].f, cands[bi].ct, cands[bi].tile, cands[bi].seg);
+    rep += "n=" + std::to_string(n) + ": " + len_describe(n) + " (" + std::to_string(cands.size()) +
+           " candidates); ";
+  }
+  // schedule switches: tile order of consecutive passes, X r2c into the output
+  double best = 1e30;
+  int balt = 1, bxin = 1;
+  for (int alt = 1; alt >= 0; alt--)
+    for (int xin = 1; xin >= 0; xin--) {
+      set_tile_order(alt != 0);
+      g_x_into_out = xin != 0;
+      const double t = time_pair(S, rin, cspec, rout, s);
+      if (t < best) { best = t; balt = alt; bxin = xin; }
+    }
+  set_tile_order(balt != 0);
+  g_x_into_out = bxin != 0;
+  rep += std::string("tile order ") + (balt ? "alternating" : "fixed") + "; X r2c " +
+         (bxin ? "into the output" : "into the work buffer") + "; fwd+bwd " + std::to_string(best) + " ms";
+  g_plan_report = rep;
+  check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+  (void)hipFree(rin);
+  (void)hipFree(rout);
+  (void)hipFree(cspec);
+}
+
 void release_state() {
   timers_resolve();
   share_release();
@@ -698,6 +806,7 @@ void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int
   get_len(g.nx);
   get_len(g.ny);
   get_len(g.nz);
+  if (g_plan_mode == 1) measure_plans(g);
   if (fuse_xy_enabled() && !g.stride1 && g.nxhpc == g.nx / 2 + 1 && g.nyc == g.ny)
     g.fuse = fuse_plan_create(g.nx, g.ny, g.nz);
   g.set = true;
@@ -956,6 +1065,47 @@ void p3dfft_amd_set_specialised(int on) { set_specialised(on != 0); }
 void p3dfft_amd_set_tile_order(int alternate) { set_tile_order(alternate != 0); }
 void p3dfft_amd_set_wide_offsets(int on) { set_wide_offsets(on != 0); }
 void p3dfft_amd_set_x_into_out(int on) { g_x_into_out = (on != 0); }
+
+// 0 (default): the default plan of every length; 1: from the next setup on,
+// p3dfft_setup measures candidate plans (radix schedule, specialised or
+// generic kernels, tile shape) of each length and the schedule switches on
+// its grid and keeps the fastest (plans persist for the process).
+void p3dfft_amd_set_plan_mode(int mode) { g_plan_mode = mode == 1 ? 1 : 0; }
+
+// The choices of the last measured setup as text; returns its length (the
+// text is truncated to len - 1 characters).
+int p3dfft_amd_plan_report(char* buf, int len) {
+  const std::string& r = g_plan_report;
+  if (buf && len > 0) {
+    const int n = std::min((int)r.size(), len - 1);
+    std::memcpy(buf, r.data(), (size_t)n);
+    buf[n] = 0;
+  }
+  return (int)r.size();
+}
+
+// Back to the default plans of every length and the default schedule
+// switches (no setup may be active).
+int p3dfft_amd_reset_plans(void) {
+  if (g.set) {
+    set_error("p3dfft_amd_reset_plans: call p3dfft_clean first");
+    return -1;
+  }
+  if (g_live_graphs > 0) {
+    set_error("p3dfft_amd_reset_plans: destroy the Poisson graphs first");
+    return -1;
+  }
+  try {
+    free_lens();
+    set_tile_order(true);
+    g_x_into_out = true;
+    g_plan_report.clear();
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return -1;
+  }
+}
 
 // 1: fused X/Y passes for the next setups, where the shape has them;
 // 0 (default): separate passes.

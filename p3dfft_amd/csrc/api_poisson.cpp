@@ -10,6 +10,7 @@
 // Solves take their work area from the caller, so any number of solves of
 // any shapes can be in flight on different streams.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -60,6 +61,9 @@ __global__ void k_fill_normal(T* p, long long n, uint64_t seed) {
 // fused Z forward / multiply / Z inverse kernel (default on); off composes
 // the solve from the forward transform, a multiply pass and the backward one
 static bool g_fused_solve = true;
+
+// Poisson graphs alive (their captured launches hold plan twiddle tables)
+std::atomic<int> g_live_graphs{0};
 
 size_t poisson_work_bytes(int nx, int ny, int nz) {
   // the spectrum: every pass after X r2c runs in place on it
@@ -241,6 +245,7 @@ int p3dfft_poisson_graph_create(int n, const int* shapes, p3amd::real_t* const* 
     g->exec = exec;
     g->nsolves = n;
     *graph_out = g;
+    g_live_graphs++;
     graph = nullptr;
     exec = nullptr;
     rc = 0;
@@ -284,6 +289,7 @@ int p3dfft_poisson_graph_destroy(void* graph) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
+  g_live_graphs--;
   return 0;
 }
 

@@ -936,16 +936,20 @@ static bool is_templated(int r) {
 
 static std::mutex g_len_mu;
 static std::map<int, Len1D> g_lens;
+static std::vector<tw_t*> g_retired_tw;
 static void prep_kernels(const Len1D& L);
 
-const Len1D& get_len(int n) {
-  std::lock_guard<std::mutex> lk(g_len_mu);
-  auto it = g_lens.find(n);
-  if (it != g_lens.end()) return it->second;
+std::vector<int> default_factors(int n) { return n > 1 ? factorize(n) : std::vector<int>{}; }
+
+// plan of length n with the radix schedule f (stage s has radix f[s]):
+// stage descriptors and the FP64-generated twiddle table
+static Len1D build_len(int n, const std::vector<int>& f) {
   Len1D L;
   std::memset(&L.d, 0, sizeof(L.d));
   L.d.n = n;
-  std::vector<int> f = n > 1 ? factorize(n) : std::vector<int>{};
+  long long prod = 1;
+  for (int r : f) prod *= r;
+  if ((n > 1 && prod != n) || (n <= 1 && !f.empty())) throw Error{"radix schedule does not multiply to the length", -1};
   if ((int)f.size() > kMaxStages) throw Error{"transform length has too many factors", -1};
   std::vector<int2> idx;
   int ns = 1;
@@ -980,14 +984,41 @@ const Len1D& get_len(int n) {
     check_hip(hipFree(didx), "hipFree");
   }
   prep_kernels(L);
-  auto res = g_lens.emplace(n, L);
+  return L;
+}
+
+const Len1D& get_len(int n) {
+  std::lock_guard<std::mutex> lk(g_len_mu);
+  auto it = g_lens.find(n);
+  if (it != g_lens.end()) return it->second;
+  auto res = g_lens.emplace(n, build_len(n, default_factors(n)));
   return res.first->second;
+}
+
+void set_len_plan(int n, const std::vector<int>& factors, bool ct_ok, int tile_bytes, int col_seg) {
+  std::lock_guard<std::mutex> lk(g_len_mu);
+  Len1D L = build_len(n, factors);
+  L.ct_ok = ct_ok;
+  L.tile_bytes = tile_bytes;
+  L.col_seg = col_seg;
+  auto it = g_lens.find(n);
+  if (it == g_lens.end()) {
+    g_lens.emplace(n, L);
+    return;
+  }
+  // the old table is retired, not freed: launches captured earlier (graphs)
+  // may still read it; retired tables are freed with the plans
+  if (it->second.tw) g_retired_tw.push_back(it->second.tw);
+  it->second = L;  // same map node: references to the entry stay valid
 }
 
 void free_lens() {
   std::lock_guard<std::mutex> lk(g_len_mu);
+  (void)hipDeviceSynchronize();
   for (auto& kv : g_lens)
     if (kv.second.tw) (void)hipFree(kv.second.tw);
+  for (tw_t* t : g_retired_tw) (void)hipFree(t);
+  g_retired_tw.clear();
   g_lens.clear();
 }
 
@@ -1271,6 +1302,7 @@ struct CtLen {
 // Specialised kernel set for a length, if one exists and its radix schedule
 // matches the length plan (the twiddle tables are laid out by that schedule).
 static const CtInfo* find_ct(const Len1D& L) {
+  if (!L.ct_ok) return nullptr;
   static const std::map<int, CtInfo> tab = {
       {32, CtLen<8, 4>::info()},       {48, CtLen<4, 4, 3>::info()},    {64, CtLen<8, 8>::info()},
       {96, CtLen<8, 4, 3>::info()},    {128, CtLen<8, 4, 4>::info()},   {160, CtLen<8, 4, 5>::info()},
@@ -1284,6 +1316,25 @@ static const CtInfo* find_ct(const Len1D& L) {
   for (int i = 0; i < c.nr; i++)
     if (c.radix[i] != L.d.radix[i]) return nullptr;
   return &c;
+}
+
+bool len_has_ct(int n) {
+  Len1D probe;
+  std::memset(&probe.d, 0, sizeof(probe.d));
+  const std::vector<int> f = default_factors(n);
+  probe.d.n = n;
+  probe.d.nst = (int)f.size();
+  for (size_t i = 0; i < f.size() && i < (size_t)kMaxStages; i++) probe.d.radix[i] = f[i];
+  return find_ct(probe) != nullptr;
+}
+
+std::string len_describe(int n) {
+  const Len1D& L = get_len(n);
+  std::string r = "radix ";
+  for (int s = 0; s < L.d.nst; s++) r += (s ? "x" : "") + std::to_string(L.d.radix[s]);
+  if (find_ct(L)) return r + ", specialised";
+  return r + ", generic, tile " + std::to_string(L.tile_bytes / 1024) + " KiB, segment " +
+         std::to_string(L.col_seg) + " B";
 }
 
 template <class F>
@@ -1319,8 +1370,8 @@ bool tile_order_alternate() { return g_alt_order; }
 void set_specialised(bool on) { g_use_ct = on; }
 
 // row-pair count per X tile
-static long long x_pairs_per_tile(int nx, long long pairs) {
-  const long long tgt = (long long)(kTileBytes / sizeof(cplx_t));
+static long long x_pairs_per_tile(const Len1D& L, int nx, long long pairs) {
+  const long long tgt = (long long)(L.tile_bytes / sizeof(cplx_t));
   long long B = std::max<long long>(1, tgt / nx);
   // keep at least ~2048 workgroups in flight when the problem allows
   B = std::min<long long>(B, std::max<long long>(1, pairs / 2048));
@@ -1329,7 +1380,7 @@ static long long x_pairs_per_tile(int nx, long long pairs) {
 
 static void x_setup(XArgs& a, const Len1D& L, long long nrows, int nx, size_t* bytes, int* nt, unsigned* nwg) {
   const long long pairs = (nrows + 1) / 2;
-  long long B = x_pairs_per_tile(nx, pairs);
+  long long B = x_pairs_per_tile(L, nx, pairs);
   const int cpt = len_cap_per_thread(L);
   while (B > 1 && B * nx > 1024LL * cpt) B--;
   a.B = (int)B;
@@ -1552,9 +1603,10 @@ void launch_col(const ColSpec& c, hipStream_t st) {
     return;
   }
   const int cpt = L ? len_cap_per_thread(*L) : kEPT;
-  // tile width: one 128-byte segment per row where capacity allows
-  int TX = (int)(128 / sizeof(cplx_t));
-  const long long tgt = (long long)(kTileBytes / sizeof(cplx_t));
+  // tile width: one segment of adjacent columns (128 bytes unless the plan
+  // says otherwise) per row where capacity allows
+  int TX = std::max(1, (int)((L ? L->col_seg : 128) / (int)sizeof(cplx_t)));
+  const long long tgt = (long long)((L ? (size_t)L->tile_bytes : kTileBytes) / sizeof(cplx_t));
   while (TX > 1 && (long long)TX * rows > std::max<long long>(tgt, rows)) TX /= 2;
   while (TX > 1 && (long long)TX * rows > 1024LL * cpt) TX /= 2;
   while (TX > 1 && TX / 2 >= c.n_inner) TX /= 2;

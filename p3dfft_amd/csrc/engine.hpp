@@ -34,11 +34,26 @@ struct Len1D {
   LenDesc d;
   tw_t* tw = nullptr;  // device table: stage twiddles then (if needed) W_n^e
   bool has_generic = false;
+  // plan choices (plan-time measurement may change them): specialised
+  // kernels allowed; data bytes per tile and bytes of adjacent columns per
+  // row segment of the run-time-generic kernels
+  bool ct_ok = true;
+  int tile_bytes = 64 * 1024;
+  int col_seg = 128;
 };
 
 // Length cache (one per library instance).
 const Len1D& get_len(int n);
 void free_lens();
+// the default radix schedule of a length
+std::vector<int> default_factors(int n);
+// replace the plan of length n (radix schedule and tile choices); the device
+// is synchronised before the old twiddle table is freed
+void set_len_plan(int n, const std::vector<int>& factors, bool ct_ok, int tile_bytes, int col_seg);
+// a specialised kernel set exists for length n with its default schedule
+bool len_has_ct(int n);
+// "radix 8x8x8, specialised" / "radix 5x5x5x8, generic, tile 64 KiB, segment 128 B"
+std::string len_describe(int n);
 
 // Largest transform length the engine runs in one workgroup.
 int max_len();

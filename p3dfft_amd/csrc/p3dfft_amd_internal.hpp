@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,8 @@ struct State {
 };
 
 extern State g;
+// Poisson graphs alive (plans may not be reset while any exists)
+extern std::atomic<int> g_live_graphs;
 
 bool fatal(const std::string& msg);
 void notice(const std::string& msg);

@@ -112,3 +112,12 @@ def test_poisson_graph_rejects_invalid_batches_without_gpu():
     assert h.value is None
     assert L.p3dfft_poisson_graph_destroy(None) == 0
     L.p3dfft_amd_clear_error()
+
+
+def test_plan_report_empty_before_any_measurement():
+    """No measured setup yet: the plan report is empty."""
+    import ctypes
+    L = p3dfft_amd.load(8)
+    buf = ctypes.create_string_buffer(64)
+    assert L.p3dfft_amd_plan_report(buf, 64) == 0
+    assert buf.value == b""

@@ -567,3 +567,32 @@ def _in_place_backward(nx, ny, nz, f):
             L.p3dfft_amd_free(buf)
         assert L.p3dfft_amd_fused_xy_status() == 0
     assert rel(back, out_of_place) < TOL[4] / 10
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+@pytest.mark.parametrize("shape", [(64, 48, 40), (60, 36, 50)])
+def test_measured_plans_match_oracle(precision, shape):
+    """Plan-time measurement (the counterpart of the reference's FFTW_MEASURE)
+    chooses per length among the specialised kernels, generic kernels with
+    three tile sizes and three column segments, and other radix orders, then
+    the schedule switches; whatever it keeps computes the same transforms."""
+    import ctypes
+    nx, ny, nz = shape
+    f = np.random.default_rng(47).standard_normal((nz, ny, nx))
+    L = p3dfft_amd.load(precision)
+    L.p3dfft_amd_set_plan_mode(1)
+    try:
+        with lib_setup(precision, nx, ny, nz) as P:
+            s = P.ftran_r2c(f, "fft")
+            b = P.btran_c2r(s, "tff")
+        buf = ctypes.create_string_buffer(2048)
+        assert L.p3dfft_amd_plan_report(buf, 2048) > 0
+        rep = buf.value.decode()
+        for n in set(shape):
+            assert "n=%d:" % n in rep, rep
+    finally:
+        L.p3dfft_amd_set_plan_mode(0)
+        assert L.p3dfft_amd_reset_plans() == 0
+    ref = oplan(nx, ny, nz).ftran_r2c(f)
+    assert rel(s, ref) < TOL[precision]
+    assert np.abs(b / (nx * ny * nz) - f).max() < PREC[precision] * nx * ny * nz * 0.25
