@@ -119,6 +119,16 @@ int p3dfft_btran_c2r_many_dev(p3dfft_real *in, long long dim_in, p3dfft_real *ou
 int p3dfft_cheby_many_dev(const p3dfft_real *in, long long dim_in, p3dfft_real *out, long long dim_out, int nv,
                           p3dfft_real Lz, void *stream);
 
+/* ---- power spectrum (the sample drivers' compute_spectrum,
+ * sample/C/driver_spec.c:298-383 and sample/FORTRAN/driver_spec.F90:263-340,
+ * as a library routine).  B: the current setup's spectrum in the conf-2
+ * layout; E[ik], ik = 0..kmax: the sum of k2*|B|^2 over stored elements with
+ * ik = int(sqrt(k2)+1/2), k2 = kx^2+ky^2+kz^2 (kx the stored x index, ky and
+ * kz folded to n-j above n/2), accumulated in FP64 in both builds.  Returns 0,
+ * or -1 when 4*kmax^2 is below the sum of (size-1)^2 over the three axes. */
+int p3dfft_power_spectrum_dev(const p3dfft_real *B, int kmax, double *E, void *stream);
+int p3dfft_power_spectrum(const p3dfft_real *B, int kmax, double *E);
+
 /* ---- share mode: one rank of a P1 x P2 grid (no counterpart entry points in
  * the reference; they split build/ftran.F90:489-780 and build/btran.F90:396-679
  * at the MPI_Alltoallv calls of fcomm1/fcomm2/bcomm1/bcomm2).  Buffers are

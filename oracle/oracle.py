@@ -248,3 +248,61 @@ def dst1_naive(x):
     out = np.zeros_like(a)
     lib().orc_dst1_naive(_dp(a), _dp(out), a.size)
     return out
+
+
+# ---- power spectrum (sample/C/driver_spec.c:298-383; the FORTRAN driver's
+# compute_spectrum, sample/FORTRAN/driver_spec.F90:263-340, is the same sum
+# except that for odd ng its 1-based test `y > ng/2` folds one index earlier)
+def _fold(j, n):
+    """|wavenumber| of storage index j on an axis of n points (driver_spec.c:336-344)."""
+    j = np.asarray(j)
+    return np.where(j > n // 2, n - j, j)
+
+
+def power_spectrum(spec, ng, kmax, stride1=False):
+    """E[ik] = sum over the stored half spectrum of k2*|B|^2, ik = int(sqrt(k2)+0.5),
+    k2 = kx^2 + ky^2 + kz^2 with kx the stored x index and ky, kz folded (FP64).
+    spec is shaped as this module's spectra: (nzc, nyc, nxhpc), or
+    (nxhpc, nyc, nzc) under STRIDE1; ng = (nx, ny, nz).  Terms with ik > kmax
+    are dropped (the reference would index past its array)."""
+    B = np.asarray(spec, dtype=np.complex128)
+    nx, ny, nz = ng
+    if stride1:
+        s2, s1, s0 = B.shape  # x slowest, z fastest
+        kx = np.arange(s2)[:, None, None]
+        ky = _fold(np.arange(s1), ny)[None, :, None]
+        kz = _fold(np.arange(s0), nz)[None, None, :]
+    else:
+        s2, s1, s0 = B.shape  # z slowest, x fastest
+        kz = _fold(np.arange(s2), nz)[:, None, None]
+        ky = _fold(np.arange(s1), ny)[None, :, None]
+        kx = np.arange(s0)[None, None, :]
+    k2 = (kx * kx + ky * ky + kz * kz).astype(np.int64)
+    ik = (np.sqrt(k2.astype(np.float64)) + 0.5).astype(np.int64)
+    w = k2 * (B.real ** 2 + B.imag ** 2)
+    keep = ik <= kmax
+    return np.bincount(ik[keep].ravel(), weights=w[keep].ravel(), minlength=kmax + 1)[: kmax + 1]
+
+
+def power_spectrum_loops(spec, ng, kmax):
+    """The driver's loop nest element by element (standard layout), for tiny
+    arrays: the brute-force check of power_spectrum."""
+    B = np.asarray(spec, dtype=np.complex128)
+    nzc, nyc, nxh = B.shape
+    E = [0.0] * (kmax + 1)
+    for z in range(nzc):
+        kz = z if z <= ng[2] // 2 else ng[2] - z
+        for y in range(nyc):
+            ky = y if y <= ng[1] // 2 else ng[1] - y
+            for x in range(nxh):
+                k2 = x * x + ky * ky + kz * kz
+                ik = int(k2 ** 0.5 + 0.5)
+                if ik <= kmax:
+                    v = B[z, y, x]
+                    E[ik] += k2 * (v.real * v.real + v.imag * v.imag)
+    return np.array(E)
+
+
+def spectrum_kmax(nx, ny, nz):
+    """The drivers' shell count (driver_spec.c:226): sqrt(nx^2+ny^2+nz^2)/2 + 1/2."""
+    return int((nx * nx + ny * ny + nz * nz) ** 0.5 * 0.5 + 0.5)

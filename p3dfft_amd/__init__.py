@@ -90,6 +90,10 @@ def load(precision=8):
     L.p3dfft_amd_set_x_into_out.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_fused_xy.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_plan_mode.argtypes = [ctypes.c_int]
+    L.p3dfft_power_spectrum.argtypes = [_vp, ctypes.c_int, _vp]
+    L.p3dfft_power_spectrum.restype = ctypes.c_int
+    L.p3dfft_power_spectrum_dev.argtypes = [_vp, ctypes.c_int, _vp, _vp]
+    L.p3dfft_power_spectrum_dev.restype = ctypes.c_int
     L.p3dfft_amd_plan_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.p3dfft_amd_plan_report.restype = ctypes.c_int
     L.p3dfft_amd_reset_plans.restype = ctypes.c_int
@@ -222,6 +226,19 @@ class P3DFFT:
         self.lib.p3dfft_ftran_r2c(a.ctypes.data, out.ctypes.data, _op(op))
         self._raise_if_error()
         return out
+
+    def power_spectrum(self, spec, kmax):
+        """The drivers' k2-weighted shell sums (sample/C/driver_spec.c:298-383)
+        of a host spectrum in this setup's layout: kmax+1 FP64 values."""
+        b = np.ascontiguousarray(spec, dtype=self.cdtype).reshape(self.spec_shape())
+        E = np.zeros(int(kmax) + 1, dtype=np.float64)
+        if self.lib.p3dfft_power_spectrum(b.ctypes.data, int(kmax), E.ctypes.data):
+            self._raise_if_error(force=True)
+        return E
+
+    def power_spectrum_dev(self, spec_ptr, kmax, e_ptr, stream=None):
+        if self.lib.p3dfft_power_spectrum_dev(spec_ptr, int(kmax), e_ptr, stream):
+            self._raise_if_error(force=True)
 
     def ftran_r2c_1d(self, a):
         """p3dfft_ftran_r2c_1d (build/ftran.F90:787-814): X r2c only, all

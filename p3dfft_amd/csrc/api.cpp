@@ -739,6 +739,9 @@ void release_state() {
   if (g.w2) (void)hipFree(g.w2);
   fuse_plan_free(g.fuse);
   g.fuse = nullptr;
+  if (g.spec_partial) (void)hipFree(g.spec_partial);
+  g.spec_partial = nullptr;
+  g.spec_partial_bytes = 0;
   if (g.stage_a) (void)hipFree(g.stage_a);
   if (g.stage_b) (void)hipFree(g.stage_b);
   g.w1 = g.w2 = nullptr;
@@ -839,6 +842,58 @@ void p3dfft_get_dims(int* start, int* end, int* size, int* conf) {
 }
 
 // build/ftran.F90:469-485 (host arrays; copied to and from HBM inside the call)
+// The sample drivers' power spectrum (sample/C/driver_spec.c:298-383,
+// sample/FORTRAN/driver_spec.F90:263-340) of the current setup's spectrum B
+// (conf-2 layout, HBM): E[ik] = sum of k2*|B|^2 over the stored elements with
+// ik = int(sqrt(k2) + 1/2), k2 = kx^2 + ky^2 + kz^2, kx the stored x index,
+// ky and kz folded (n - j above n/2).  E: kmax+1 doubles in HBM.  The kmax
+// check is driver_spec.F90:272-275.  Asynchronous on `stream`; the first call
+// for a larger kmax allocates its per-plane scratch.
+int p3dfft_power_spectrum_dev(const real_t* B, int kmax, double* E, void* stream) {
+  if (!require_full()) return -1;
+  GUARD_BEGIN
+  int d0, d1, d2, f0, f1, f2;
+  if (g.stride1) {
+    d0 = g.nzc; d1 = g.nyc; d2 = g.nxhpc; f0 = g.nz; f1 = g.ny; f2 = 0;
+  } else {
+    d0 = g.nxhpc; d1 = g.nyc; d2 = g.nzc; f0 = 0; f1 = g.ny; f2 = g.nz;
+  }
+  const double need = (double)(d0 - 1) * (d0 - 1) + (double)(d1 - 1) * (d1 - 1) + (double)(d2 - 1) * (d2 - 1);
+  if (kmax < 0 || 4.0 * kmax * kmax < need) {
+    set_error("Error: kmax is less than it should be (" + std::to_string(kmax) + ")");
+    return -1;
+  }
+  const size_t pb = spectrum_partial_bytes(d2, kmax);
+  if (pb > g.spec_partial_bytes) {
+    if (g.spec_partial) (void)hipFree(g.spec_partial);
+    g.spec_partial = nullptr;
+    g.spec_partial_bytes = 0;
+    check_hip(hipMalloc(&g.spec_partial, pb), "hipMalloc(spectrum scratch)");
+    g.spec_partial_bytes = pb;
+  }
+  launch_spectrum((const cplx_t*)B, d0, d1, d2, f0, f1, f2, kmax, g.spec_partial, E, (hipStream_t)stream);
+  return 0;
+  GUARD_END
+  return -1;
+}
+
+// The same on a host spectrum, E returned to the host.
+int p3dfft_power_spectrum(const real_t* B, int kmax, double* E) {
+  if (!require_full()) return -1;
+  GUARD_BEGIN
+  ensure_staging();
+  double* dE = nullptr;
+  check_hip(hipMalloc(&dE, (size_t)(kmax + 1) * sizeof(double)), "hipMalloc(spectrum)");
+  check_hip(hipMemcpy(g.stage_b, B, spec_elems() * sizeof(cplx_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  const int rc = p3dfft_power_spectrum_dev((const real_t*)g.stage_b, kmax, dE, g.stream);
+  if (rc == 0)
+    check_hip(hipMemcpy(E, dE, (size_t)(kmax + 1) * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  (void)hipFree(dE);
+  return rc;
+  GUARD_END
+  return -1;
+}
+
 void p3dfft_ftran_r2c(real_t* A, real_t* B, unsigned char* op) {
   if (!require_full()) return;
   GUARD_BEGIN

@@ -825,6 +825,99 @@ __global__ __launch_bounds__(NT, W) void k_xy_ct(XArgs xa, FuseArgs f) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// k2-weighted shell sums of a stored half spectrum (the drivers' power
+// spectrum, sample/C/driver_spec.c:298-383): element (a, b, c), a fastest,
+// adds k2*|B|^2 to shell int(sqrt(k2) + 1/2), k2 = ka^2 + kb^2 + kc^2, where an
+// axis index j folds to n - j above n/2 on axes with fold length n > 0.
+// One workgroup per index c; one histogram per wave in LDS (FP64), each
+// thread summing runs of equal shells along a before its LDS add, so the
+// sums do not depend on scheduling; the wave histograms and then the
+// workgroups' are added in a fixed order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int fold_k(int j, int n) { return (n > 0 && j > n / 2) ? n - j : j; }
+
+// exact shell of an integer k2: the r with r(r-1) < k2 <= r(r+1), which is
+// int(sqrt(k2) + 1/2) (no integer k2 lies on a half-integer radius)
+__device__ __forceinline__ int shell_of(int k2) {
+  int r = (int)(sqrtf((float)k2) + 0.5f);
+  if (r * (r + 1) < k2) r++;
+  else if (r > 0 && r * (r - 1) >= k2) r--;
+  return r;
+}
+
+// one workgroup per chunk `ch` of plane c (blockIdx.x = c * nchunk + ch)
+template <class T>
+__global__ __launch_bounds__(256) void k_spectrum(const cpx<T>* B, int d0, int d1, int f0, int f1, int f2,
+                                                  int kmax, int nchunk, double* partial) {
+  constexpr int R = 4;  // consecutive elements per thread per step
+  double* hist = (double*)g_smem;
+  const int nw = blockDim.x / 64, w = threadIdx.x / 64, nb = kmax + 1;
+  for (int i = threadIdx.x; i < nw * nb; i += blockDim.x) hist[i] = 0.0;
+  __syncthreads();
+  double* hw = hist + w * nb;
+  const int c = blockIdx.x / nchunk, ch = blockIdx.x - c * nchunk;
+  const int kc = fold_k(c, f2);
+  const int kc2 = kc * kc;
+  const cpx<T>* P = B + (long long)c * d0 * d1;
+  const unsigned n = (unsigned)d0 * (unsigned)d1;
+  // chunk bounds, multiples of one workgroup step
+  const unsigned step = blockDim.x * R;
+  const unsigned per = ((n + nchunk - 1) / nchunk + step - 1) / step * step;
+  const unsigned lo = ch * per, hi = min(n, lo + per);
+  for (unsigned i0 = lo + threadIdx.x * R; i0 < hi; i0 += step) {
+    unsigned b = i0 / (unsigned)d0, a = i0 - b * (unsigned)d0;
+    int kb = fold_k((int)b, f1);
+    int cur = -1;
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (i0 + r < hi) {
+        const cpx<T> v = P[i0 + r];
+        const int ka = fold_k((int)a, f0);
+        const int k2 = ka * ka + kb * kb + kc2;
+        const int ik = shell_of(k2);
+        const double e = (double)k2 * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+        if (ik != cur) {
+          if (cur >= 0 && cur <= kmax) atomicAdd(hw + cur, acc);
+          cur = ik;
+          acc = e;
+        } else {
+          acc += e;
+        }
+        if (++a == (unsigned)d0) {
+          a = 0;
+          b++;
+          kb = fold_k((int)b, f1);
+        }
+      }
+    }
+    if (cur >= 0 && cur <= kmax) atomicAdd(hw + cur, acc);
+  }
+  __syncthreads();
+  // bin-major partials: bin k of every workgroup contiguous
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    double sum = 0.0;
+    for (int q = 0; q < nw; q++) sum += hist[q * nb + k];
+    partial[(long long)k * gridDim.x + blockIdx.x] = sum;
+  }
+}
+
+// one workgroup of 256 per bin: strided partial sums, then a fixed tree
+__global__ __launch_bounds__(256) void k_spectrum_sum(const double* partial, int nparts, double* E) {
+  __shared__ double red[256];
+  const double* row = partial + (long long)blockIdx.x * nparts;
+  double sum = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 256) sum += row[p];
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) E[blockIdx.x] = red[0];
+}
+
 // XCC_ID of each workgroup (the fused kernels' queue selector)
 __global__ void k_xcc_id(unsigned* out, int n) {
   if (threadIdx.x == 0 && (int)blockIdx.x < n) out[blockIdx.x] = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15;
@@ -1356,6 +1449,7 @@ static void prep_kernels(const Len1D& L) {
     prep_generic_nt<GoXc2r>();
     prep_generic_nt<GoCol<true>>();
     prep_generic_nt<GoCol<false>>();
+    allow_big_lds(k_spectrum<real_t>);
     generic_done = true;
   }
   if (const CtInfo* c = find_ct(L)) c->prep();
@@ -1822,6 +1916,31 @@ void launch_xy_fwd(FusePlan* P, const real_t* in, cplx_t* out, hipStream_t st) {
 
 void launch_yx_bwd(FusePlan* P, const cplx_t* in, real_t* out, cplx_t* ring, hipStream_t st) {
   fuse_launch(P, false, in, out, ring, st);
+}
+
+// workgroups per plane: at least ~2048 workgroups in all, up to 64 per plane
+static int spectrum_chunks(int d2) { return std::max(1, std::min(64, (2048 + d2 - 1) / std::max(d2, 1))); }
+
+size_t spectrum_partial_bytes(int d2, int kmax) {
+  return (size_t)d2 * spectrum_chunks(d2) * (kmax + 1) * sizeof(double);
+}
+
+void launch_spectrum(const cplx_t* B, int d0, int d1, int d2, int f0, int f1, int f2, int kmax, double* partial,
+                     double* E, hipStream_t st) {
+  const size_t lds = 4 * (size_t)(kmax + 1) * sizeof(double);
+  if (lds > kLdsMax) throw Error{"power spectrum: too many shells for one workgroup's LDS", -1};
+  if ((long long)d0 * d1 >= (1LL << 31)) throw Error{"power spectrum: plane too large", -1};
+  // k2 in 32 bits: every index, folded or not, is below 2^14
+  if (std::max({d0, d1, d2, f0, f1, f2}) >= (1 << 14)) throw Error{"power spectrum: axis too long", -1};
+  const int nchunk = spectrum_chunks(d2);
+  const long long nwg = (long long)d2 * nchunk;
+  if (nwg > 0x7fffffffLL) throw Error{"power spectrum: grid too large", -1};
+  if (nwg > 0) {
+    k_spectrum<real_t><<<(unsigned)nwg, 256, lds, st>>>(B, d0, d1, f0, f1, f2, kmax, nchunk, partial);
+    check_hip(hipGetLastError(), "k_spectrum launch");
+  }
+  k_spectrum_sum<<<kmax + 1, 256, 0, st>>>(partial, (int)nwg, E);
+  check_hip(hipGetLastError(), "k_spectrum_sum launch");
 }
 
 void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long ks, double scale, double L,

@@ -130,6 +130,12 @@ void launch_xy_fwd(FusePlan* P, const real_t* in, cplx_t* out, hipStream_t st);
 // backward: Y of in (nx/2+1, ny, nz) through `ring`, then X c2r into out;
 // in is left unchanged and must not overlap out
 void launch_yx_bwd(FusePlan* P, const cplx_t* in, real_t* out, cplx_t* ring, hipStream_t st);
+// k2-weighted shell sums of a (d0, d1, d2) half spectrum (a fastest; fold
+// lengths f0, f1, f2, 0 = the axis is not folded) into E[0..kmax], through
+// `partial` (spectrum_partial_bytes)
+size_t spectrum_partial_bytes(int d2, int kmax);
+void launch_spectrum(const cplx_t* B, int d0, int d1, int d2, int f0, int f1, int f2, int kmax, double* partial,
+                     double* E, hipStream_t st);
 void set_fuse_xy(bool on);
 bool fuse_xy_enabled();
 

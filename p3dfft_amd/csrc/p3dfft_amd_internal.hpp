@@ -34,6 +34,9 @@ struct State {
   int nv_alloc = 0;  // variables the work buffers hold (grown by the _many calls)
   // fused X/Y passes of this grid (null when the shape has none)
   FusePlan* fuse = nullptr;
+  // per-plane partial shell sums of the power spectrum (grown on demand)
+  double* spec_partial = nullptr;
+  size_t spec_partial_bytes = 0;
   // share mode (one rank of a P1 x P2 grid): Z pencil work buffer and splits
   bool share = false;
   cplx_t* w3 = nullptr;

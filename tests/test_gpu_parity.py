@@ -596,3 +596,27 @@ def test_measured_plans_match_oracle(precision, shape):
     ref = oplan(nx, ny, nz).ftran_r2c(f)
     assert rel(s, ref) < TOL[precision]
     assert np.abs(b / (nx * ny * nz) - f).max() < PREC[precision] * nx * ny * nz * 0.25
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape,stride1", [((64, 64, 64), False), ((48, 40, 30), False), ((36, 20, 26), True),
+                                           ((128, 96, 80), False), ((31, 17, 23), False)])
+def test_power_spectrum_matches_oracle(precision, shape, stride1):
+    """The drivers' power spectrum (sample/C/driver_spec.c:298-383) of the
+    library's own spectrum, against the oracle's restatement on the same
+    stored values: shells summed in FP64 in both, so they differ only by
+    summation order; repeated calls are bit-identical; a kmax below the
+    drivers' bound is refused (driver_spec.F90:272-275)."""
+    nx, ny, nz = shape
+    f = np.random.default_rng(53).standard_normal((nz, ny, nx))
+    kmax = O.spectrum_kmax(nx, ny, nz)
+    with lib_setup(precision, nx, ny, nz, stride1=stride1) as P:
+        s = P.ftran_r2c(f, "fft")
+        E = P.power_spectrum(s, kmax)
+        E2 = P.power_spectrum(s, kmax)
+        with pytest.raises(p3dfft_amd.P3DFFTError, match="kmax is less than it should be"):
+            P.power_spectrum(s, 1)
+    ref = O.power_spectrum(s.astype(np.complex128), (nx, ny, nz), kmax, stride1=stride1)
+    assert np.array_equal(E, E2)
+    assert np.abs(E - ref).max() <= 1e-13 * ref.sum()
+    assert E.sum() > 0

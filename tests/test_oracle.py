@@ -391,3 +391,42 @@ def test_share_maps_with_y_pruning():
         assert rel(spec[(i, j)], ref) < RTOL64
     ref_back, _ = O.Plan(nx, ny, nz, nyc=nyc).btran_c2r(full)
     assert rel(field, ref_back) < RTOL64
+
+
+# ---------------------------------------------------------------- power spectrum
+def test_power_spectrum_single_mode_closed_form():
+    """One stored mode B(x, y, z) = A: E is k2*|A|^2 at shell int(|k| + 1/2)
+    and zero elsewhere; y and z above n/2 fold to n - j
+    (sample/C/driver_spec.c:336-344)."""
+    nx, ny, nz = 16, 12, 10
+    kmax = O.spectrum_kmax(nx, ny, nz)
+    for (x, y, z) in [(3, 2, 1), (5, 9, 7), (0, 0, 0), (8, 6, 5)]:
+        B = np.zeros((nz, ny, nx // 2 + 1), complex)
+        B[z, y, x] = 2.0 - 1.5j
+        ky = y if y <= ny // 2 else ny - y
+        kz = z if z <= nz // 2 else nz - z
+        k2 = x * x + ky * ky + kz * kz
+        E = O.power_spectrum(B, (nx, ny, nz), kmax)
+        want = np.zeros(kmax + 1)
+        want[int(np.sqrt(k2) + 0.5)] = k2 * 6.25
+        assert np.array_equal(E, want), (x, y, z)
+
+
+def test_power_spectrum_matches_loop_restatement_and_total():
+    """The vectorised restatement equals the drivers' loop nest element by
+    element on a small random spectrum, and its shells add up to the total
+    sum of k2*|B|^2; the STRIDE1 form of the same array gives the same E."""
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 10, 7, 9
+    kmax = O.spectrum_kmax(nx, ny, nz)
+    B = rng.standard_normal((nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((nz, ny, nx // 2 + 1))
+    E = O.power_spectrum(B, (nx, ny, nz), kmax)
+    L = O.power_spectrum_loops(B, (nx, ny, nz), kmax)
+    assert np.allclose(E, L, rtol=1e-13, atol=0)
+    x = np.arange(nx // 2 + 1)[None, None, :]
+    y = np.minimum(np.arange(ny), ny - np.arange(ny))[None, :, None]
+    z = np.minimum(np.arange(nz), nz - np.arange(nz))[:, None, None]
+    total = ((x * x + y * y + z * z) * np.abs(B) ** 2).sum()
+    assert abs(E.sum() - total) < 1e-12 * total
+    Es = O.power_spectrum(np.ascontiguousarray(B.transpose(2, 1, 0)), (nx, ny, nz), kmax, stride1=True)
+    assert np.allclose(Es, E, rtol=1e-13, atol=0)
