@@ -113,13 +113,30 @@ def run_precision(precision, n, f, steps, warmup, chunk_bytes=None, timers=True)
         # one more forward for the accuracy comparison
         P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft")
         s = spec.download(P.spec_shape(), P.cdtype)
+        # the drivers' power spectrum of that spectrum (HBM in, HBM out)
+        kmax = int((3.0 * n * n) ** 0.5 * 0.5 + 0.5)
+        ebuf = P.alloc((kmax + 1) * 8)
+        for _ in range(3):
+            P.power_spectrum_dev(spec.ptr, kmax, ebuf.ptr)
+        P.synchronize()
+        e0 = lib.p3dfft_amd_event_create()
+        e1 = lib.p3dfft_amd_event_create()
+        lib.p3dfft_amd_event_record(e0, None)
+        for _ in range(10):
+            P.power_spectrum_dev(spec.ptr, kmax, ebuf.ptr)
+        lib.p3dfft_amd_event_record(e1, None)
+        P.synchronize()
+        spec_ms = lib.p3dfft_amd_event_elapsed_ms(e0, e1) / 10
+        lib.p3dfft_amd_event_destroy(e0)
+        lib.p3dfft_amd_event_destroy(e1)
+        ebuf.free()
         P.btran_c2r_dev(spec.ptr, out.ptr, "tff")
         back = out.download(P.real_shape(), P.rdtype)
         fin.free(); spec.free(); out.free()
     finally:
         P.clean()
     return {"ms_per_step": ev_ms / steps, "wall_ms_per_step": wall * 1e3 / steps, "timers": timers,
-            "spec": s, "back": back}
+            "spec": s, "back": back, "spectrum_ms": spec_ms}
 
 
 def copy_peak(lib, nbytes=2 << 30, reps=10):
@@ -388,6 +405,10 @@ def main():
                  "copy_kernel_GBps": cpk / 1e9, "frac_of_copy_peak": 2 * B / t / cpk,
                  "gflops_5NlogN": 2 * 5 * nr * np.log2(nr) / t / 1e9, "wall_ms_per_step": r32["wall_ms_per_step"]},
         "kernels": kern,
+        "power_spectrum": {"ms": r32["spectrum_ms"], "shells": int((3.0 * n * n) ** 0.5 * 0.5 + 0.5) + 1,
+                           "spectrum_read_GBps": (n // 2 + 1) * n * n * 8 / (r32["spectrum_ms"] * 1e-3) / 1e9,
+                           "what": "k2-weighted shell sums of the FP32 spectrum (sample/C/driver_spec.c "
+                                   "compute_spectrum), HBM in and out"},
     }
     if r64 is not None:
         ref = r64["spec"].astype(np.complex128)
