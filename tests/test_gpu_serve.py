@@ -206,3 +206,40 @@ def test_graph_create_rejects_small_work_area():
         S.arena.release(blk)
     finally:
         S.close()
+
+
+def test_graph_survives_plan_measurement():
+    """A plan measurement that replaces the plans of a graph's lengths retires
+    their twiddle tables instead of freeing them: the graph captured before it
+    still replays to the same fields, and plans reset only once no graph is
+    alive."""
+    shapes = [(48, 40, 36), (64, 60, 50)]
+    S = serve.PoissonServer(shapes, precision=4, slots=2)
+    S.arena.close()
+    S.arena = serve.Arena(S.lib, sum((w + 255) & ~255 for w in S.work))
+    idx = [0, 1]
+    G = None
+    L = S.lib
+    try:
+        S.fill()
+        S.run_streams(idx, reps=1)
+        want = [S.download(i) for i in idx]
+        G = S.graph(idx, nstreams=2)
+        P = p3dfft_amd.P3DFFT(4)
+        L.p3dfft_amd_set_plan_mode(1)
+        try:
+            P.setup((1, 1), 48, 40, 36)  # measures new plans for 48, 40 and 36
+        finally:
+            L.p3dfft_amd_set_plan_mode(0)
+            P.clean()
+        assert L.p3dfft_amd_reset_plans() == -1  # a graph is alive
+        S.fill()
+        G.launch()
+        L.p3dfft_amd_synchronize()
+        for i in idx:
+            assert np.array_equal(S.download(i), want[i]), i
+    finally:
+        if G is not None:
+            G.close()
+        S.close()
+    assert L.p3dfft_amd_reset_plans() == 0
