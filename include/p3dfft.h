@@ -177,6 +177,14 @@ int p3dfft_poisson_graph_create(int n, const int *shapes, p3dfft_real *const *fi
                                 const size_t *work_bytes, int nstreams, void **graph);
 int p3dfft_poisson_graph_launch(void *graph, void *stream);
 int p3dfft_poisson_graph_destroy(void *graph);
+/* Capture the library's asynchronous _dev calls issued on `stream` (a created
+ * stream, not NULL) between begin and end into a HIP graph; launch replays
+ * them.  The stage timers pause during a capture; calls that allocate on
+ * first use (p3dfft_power_spectrum_dev) must have run once before it. */
+int p3dfft_amd_capture_begin(void *stream);
+int p3dfft_amd_capture_end(void *stream, void **graph);
+int p3dfft_amd_graph_launch(void *graph, void *stream);
+int p3dfft_amd_graph_destroy(void *graph);
 /* build the plan and twiddle table of a transform length ahead of time */
 int p3dfft_amd_prepare_length(int n);
 /* i.i.d. N(0,1) values from a counter-based generator keyed by seed */

@@ -90,6 +90,17 @@ def load(precision=8):
     L.p3dfft_amd_set_x_into_out.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_fused_xy.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_plan_mode.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_capture_begin.argtypes = [_vp]
+    L.p3dfft_amd_capture_begin.restype = ctypes.c_int
+    L.p3dfft_amd_capture_end.argtypes = [_vp, ctypes.POINTER(_vp)]
+    L.p3dfft_amd_capture_end.restype = ctypes.c_int
+    L.p3dfft_amd_graph_launch.argtypes = [_vp, _vp]
+    L.p3dfft_amd_graph_launch.restype = ctypes.c_int
+    L.p3dfft_amd_graph_destroy.argtypes = [_vp]
+    L.p3dfft_amd_graph_destroy.restype = ctypes.c_int
+    L.p3dfft_amd_stream_create.restype = _vp
+    L.p3dfft_amd_stream_destroy.argtypes = [_vp]
+    L.p3dfft_amd_stream_synchronize.argtypes = [_vp]
     L.p3dfft_power_spectrum.argtypes = [_vp, ctypes.c_int, _vp]
     L.p3dfft_power_spectrum.restype = ctypes.c_int
     L.p3dfft_power_spectrum_dev.argtypes = [_vp, ctypes.c_int, _vp, _vp]

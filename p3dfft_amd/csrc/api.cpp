@@ -1162,6 +1162,78 @@ int p3dfft_amd_reset_plans(void) {
   }
 }
 
+// Stream capture of the library's asynchronous (_dev) calls into a HIP
+// graph: every transform issues kernel launches only (kernel attributes are
+// set when a length is planned and the work buffers exist from setup), so a
+// captured ftran/btran sequence replays with one launch.  The stage timers
+// are off between begin and end (their events cannot be read back from a
+// graph); calls that allocate on first use (power spectrum scratch, staging
+// of host arrays) must have been made once before the capture.
+struct CapturedGraph {
+  hipGraphExec_t exec = nullptr;
+};
+static bool g_cap_timing = true;
+
+int p3dfft_amd_capture_begin(void* stream) {
+  if (!stream) {
+    set_error("p3dfft_amd_capture_begin: capture needs a created stream, not the null stream");
+    return -1;
+  }
+  const hipError_t e = hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    return -1;
+  }
+  g_cap_timing = g.timing;
+  g.timing = false;
+  return 0;
+}
+
+int p3dfft_amd_capture_end(void* stream, void** graph) {
+  g.timing = g_cap_timing;
+  if (graph) *graph = nullptr;
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &gr);
+  if (e != hipSuccess || !gr) {
+    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return -1;
+  }
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return -1;
+  }
+  CapturedGraph* c = new CapturedGraph;
+  c->exec = ex;
+  g_live_graphs++;
+  if (graph) *graph = c;
+  return 0;
+}
+
+int p3dfft_amd_graph_launch(void* graph, void* stream) {
+  if (!graph) {
+    set_error("p3dfft_amd_graph_launch: null graph");
+    return -1;
+  }
+  const hipError_t e = hipGraphLaunch(((CapturedGraph*)graph)->exec, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int p3dfft_amd_graph_destroy(void* graph) {
+  if (!graph) return 0;
+  CapturedGraph* c = (CapturedGraph*)graph;
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  delete c;
+  g_live_graphs--;
+  return 0;
+}
+
 // 1: fused X/Y passes for the next setups, where the shape has them;
 // 0 (default): separate passes.
 void p3dfft_amd_set_fused_xy(int on) { set_fuse_xy(on != 0); }

@@ -655,3 +655,57 @@ def test_c3_power_law_fp32_against_fp64(n):
     assert e_spec < 2e-6
     assert e_rt < 2e-6
     assert rel(b64, a) < 1e-14
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+@pytest.mark.parametrize("shape", [(64, 48, 40), (60, 36, 50)])
+def test_captured_transforms_replay_bit_exact(precision, shape):
+    """ftran_r2c_dev + btran_c2r_dev captured from a stream into a HIP graph
+    (kernel launches only: no allocation, attribute call or memset) replay to
+    the eager results bit for bit, three times, with eager transforms and
+    uploads between the replays."""
+    import ctypes
+    nx, ny, nz = shape
+    rng = np.random.default_rng(59)
+    f = rng.standard_normal((nz, ny, nx))
+    g2 = rng.standard_normal((nz, ny, nx))
+    with lib_setup(precision, nx, ny, nz) as P:
+        L = P.lib
+        st = L.p3dfft_amd_stream_create()
+        fin, spec, out = P.alloc(P.real_bytes()), P.alloc(P.spec_bytes()), P.alloc(P.real_bytes())
+        other, ospec = P.alloc(P.real_bytes()), P.alloc(P.spec_bytes())
+        graph = ctypes.c_void_p()
+        try:
+            fin.upload(f.astype(P.rdtype))
+            P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft", st)
+            L.p3dfft_amd_stream_synchronize(st)
+            s0 = spec.download(P.spec_shape(), P.cdtype)
+            P.btran_c2r_dev(spec.ptr, out.ptr, "tff", st)
+            L.p3dfft_amd_stream_synchronize(st)
+            b0 = out.download(P.real_shape(), P.rdtype)
+            assert L.p3dfft_amd_capture_begin(st) == 0
+            P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft", st)
+            P.btran_c2r_dev(spec.ptr, out.ptr, "tff", st)
+            assert L.p3dfft_amd_capture_end(st, ctypes.byref(graph)) == 0
+            for rep in range(3):
+                other.upload(g2.astype(P.rdtype))
+                P.ftran_r2c_dev(other.ptr, ospec.ptr, "fft", st)  # eager work between replays
+                out.upload(np.zeros((nz, ny, nx), P.rdtype))
+                assert L.p3dfft_amd_graph_launch(graph, st) == 0
+                L.p3dfft_amd_stream_synchronize(st)
+                assert np.array_equal(out.download(P.real_shape(), P.rdtype), b0), rep
+            # the spectrum the graph's ftran produced, before its btran overwrote it in place
+            assert L.p3dfft_amd_capture_begin(st) == 0
+            P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft", st)
+            g1 = ctypes.c_void_p()
+            assert L.p3dfft_amd_capture_end(st, ctypes.byref(g1)) == 0
+            assert L.p3dfft_amd_graph_launch(g1, st) == 0
+            L.p3dfft_amd_stream_synchronize(st)
+            assert np.array_equal(spec.download(P.spec_shape(), P.cdtype), s0)
+            L.p3dfft_amd_graph_destroy(g1)
+        finally:
+            if graph.value:
+                L.p3dfft_amd_graph_destroy(graph)
+            for b in (fin, spec, out, other, ospec):
+                b.free()
+            L.p3dfft_amd_stream_destroy(st)
