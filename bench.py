@@ -316,8 +316,9 @@ def serve_leg(n_req, slots, load_frac=0.7):
 
 def batch_leg(n_req=256, slots=8, reps=20):
     """A fixed batch of small mixed-size solves (axes from {32,...,128}, FP32)
-    replayed `reps` times: issued eagerly on `slots` streams, and captured once
-    into a HIP graph of `slots` branches (p3dfft_poisson_graph_*)."""
+    run `reps` times: issued eagerly on `slots` streams, captured once into a
+    HIP graph of `slots` branches (p3dfft_poisson_graph_*), and as grouped
+    launches, one per pass and transform length (p3dfft_poisson_batch_*)."""
     from p3dfft_amd import serve
     shapes = serve.make_requests(n_req, seed=2002, sizes=(32, 48, 64, 96, 128))
     S = serve.PoissonServer(shapes, precision=4, slots=slots)
@@ -332,14 +333,21 @@ def batch_leg(n_req=256, slots=8, reps=20):
         G = S.graph(idx, nstreams=slots)
         G.run(reps=2)  # warm
         t_graph = min(G.run(reps=reps) for _ in range(3))
+        Bt = S.batch(idx)
+        launches = Bt.grouped_launches
+        Bt.run(reps=2)  # warm
+        t_grp = min(Bt.run(reps=reps) for _ in range(3))
+        Bt.close()
     finally:
         if G is not None:
             G.close()
         S.close()
-    eager, graph = n_req * reps / t_eager, n_req * reps / t_graph
+    eager, graph, grp = n_req * reps / t_eager, n_req * reps / t_graph, n_req * reps / t_grp
     return {"requests": n_req, "branches": slots, "reps": reps,
             "eager_streams_solves_per_s": eager, "graph_solves_per_s": graph,
             "speedup_graph_vs_eager": graph / eager,
+            "grouped_solves_per_s": grp, "grouped_launches_per_batch": launches,
+            "speedup_grouped_vs_eager": grp / eager,
             "mean_points_per_request": float(np.mean([np.prod(s) for s in shapes]))}
 
 
@@ -442,8 +450,9 @@ def main():
                                                           line["serve"]["one_at_a_time_solves_per_s"]))
         log("batched leg (graph-captured vs eager streams)")
         line["batched"] = batch_leg(slots=args.serve_slots)
-        log("batched %.1f solves/s graph, %.1f eager" % (line["batched"]["graph_solves_per_s"],
-                                                          line["batched"]["eager_streams_solves_per_s"]))
+        log("batched %.1f solves/s grouped, %.1f graph, %.1f eager" % (
+            line["batched"]["grouped_solves_per_s"], line["batched"]["graph_solves_per_s"],
+            line["batched"]["eager_streams_solves_per_s"]))
     if not args.no_cpu:
         log("cpu baseline (oracle, %d threads)" % args.cpu_threads)
         line["cpu_baseline"] = cpu_baseline(n, f, args.cpu_threads)

@@ -177,6 +177,16 @@ int p3dfft_poisson_graph_create(int n, const int *shapes, p3dfft_real *const *fi
                                 const size_t *work_bytes, int nstreams, void **graph);
 int p3dfft_poisson_graph_launch(void *graph, void *stream);
 int p3dfft_poisson_graph_destroy(void *graph);
+/* A fixed batch of n solves (arguments as for p3dfft_poisson_graph_create) run
+ * as grouped launches: each pass of every solve whose lengths have
+ * specialised kernels is one launch per pass and transform length; the
+ * other solves run one by one after them.  Results are bit-identical to
+ * p3dfft_poisson_solve of each field.  launch is asynchronous on `stream`. */
+int p3dfft_poisson_batch_create(int n, const int *shapes, p3dfft_real *const *fields, void *const *works,
+                                const size_t *work_bytes, void **batch);
+int p3dfft_poisson_batch_launch(void *batch, void *stream);
+int p3dfft_poisson_batch_info(void *batch, int *grouped_launches, int *single_solves);
+int p3dfft_poisson_batch_destroy(void *batch);
 /* Capture the library's asynchronous _dev calls issued on `stream` (a created
  * stream, not NULL) between begin and end into a HIP graph; launch replays
  * them.  The stage timers pause during a capture; calls that allocate on

@@ -268,6 +268,96 @@ int p3dfft_poisson_graph_create(int n, const int* shapes, p3amd::real_t* const* 
   return rc;
 }
 
+// A fixed batch of solves run as grouped launches: every pass of every
+// solve whose lengths have specialised kernels is issued as one launch per
+// pass and transform length (five to a few tens of launches per batch instead
+// of five to seven per solve), each workgroup finding its field in a tile
+// table; the other solves run one by one after them.  Results are
+// bit-identical to p3dfft_poisson_solve of each field.
+struct PoissonBatch {
+  GroupBatch* gb = nullptr;
+  std::vector<int> singles, shapes;
+  std::vector<p3amd::real_t*> fields;
+  std::vector<void*> works;
+  std::vector<size_t> wbytes;
+};
+
+int p3dfft_poisson_batch_create(int n, const int* shapes, p3amd::real_t* const* fields, void* const* works,
+                                const size_t* work_bytes, void** batch_out) {
+  if (!batch_out || n <= 0 || !shapes || !fields || !works || !work_bytes) {
+    set_error("p3dfft_poisson_batch_create: invalid arguments");
+    return -1;
+  }
+  *batch_out = nullptr;
+  for (int i = 0; i < n; i++) {
+    const int nx = shapes[3 * i], ny = shapes[3 * i + 1], nz = shapes[3 * i + 2];
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > max_len() || ny > max_len() || nz > max_len()) {
+      set_error("p3dfft_poisson_batch_create: invalid grid");
+      return -1;
+    }
+    if (!fields[i] || !works[i] || work_bytes[i] < poisson_work_bytes(nx, ny, nz)) {
+      set_error("p3dfft_poisson_batch_create: missing field or work area too small");
+      return -1;
+    }
+  }
+  for (int i = 0; i < 3 * n; i++)
+    if (p3dfft_amd_prepare_length(shapes[i]) != 0) return -1;
+  PoissonBatch* b = new PoissonBatch;
+  try {
+    b->gb = group_batch_create(n, shapes, fields, works, &b->singles);
+  } catch (const Error& e) {
+    delete b;
+    set_error(e.msg);
+    return -1;
+  }
+  b->shapes.assign(shapes, shapes + 3 * n);
+  b->fields.assign(fields, fields + n);
+  b->works.assign(works, works + n);
+  b->wbytes.assign(work_bytes, work_bytes + n);
+  g_live_graphs++;
+  *batch_out = b;
+  return 0;
+}
+
+// Issue the batch on `stream` (asynchronous; in place on the fields).
+int p3dfft_poisson_batch_launch(void* batch, void* stream) {
+  if (!batch) {
+    set_error("p3dfft_poisson_batch_launch: null batch");
+    return -1;
+  }
+  PoissonBatch* b = (PoissonBatch*)batch;
+  try {
+    group_batch_launch(b->gb, (hipStream_t)stream);
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return -1;
+  }
+  for (int i : b->singles)
+    if (p3dfft_poisson_solve(b->shapes[3 * i], b->shapes[3 * i + 1], b->shapes[3 * i + 2], b->fields[i],
+                             b->works[i], b->wbytes[i], stream) != 0)
+      return -1;
+  return 0;
+}
+
+// Launches one p3dfft_poisson_batch_launch issues for the grouped solves,
+// and how many solves run one by one after them.
+int p3dfft_poisson_batch_info(void* batch, int* grouped_launches, int* single_solves) {
+  if (!batch) return -1;
+  PoissonBatch* b = (PoissonBatch*)batch;
+  if (grouped_launches) *grouped_launches = group_batch_launches(b->gb);
+  if (single_solves) *single_solves = (int)b->singles.size();
+  return 0;
+}
+
+int p3dfft_poisson_batch_destroy(void* batch) {
+  if (!batch) return 0;
+  PoissonBatch* b = (PoissonBatch*)batch;
+  group_batch_free(b->gb);
+  delete b;
+  g_live_graphs--;
+  return 0;
+}
+
 // Replay every captured solve on `stream` (asynchronous; the fields are
 // solved in place again, so the caller refreshes them between replays).
 int p3dfft_poisson_graph_launch(void* graph, void* stream) {

@@ -20,7 +20,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <vector>
 
 #include "engine.hpp"
 #include "engine_ct.hpp"
@@ -634,6 +636,81 @@ __global__ __launch_bounds__(NT, W) void k_zsolve_ct(ColArgs a, double scale, in
   const ColIO io = col_io<T, TX>(a, c0, nb);
   SolveCtx sc;
   sc.nxh = nxh; sc.ny = ny; sc.nz = a.L.n; sc.scale = scale;
+  run_fft_solve<T, TX, NT>(lds, tw, io, c0, sc, RL{});
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches: one pass of many arrays of one transform length (the
+// fields of a batch of solves of mixed shapes) in one launch.  Each
+// workgroup finds its array in a tile -> array table and runs the same tile
+// body as the single-array kernel, so results are bit-identical to solving
+// the arrays one by one.
+// ---------------------------------------------------------------------------
+struct GrpDesc {
+  const void* in;
+  void* out;
+  long long nrows;   // X: rows; columns: outer count
+  long long sl, so;  // columns: row and outer strides (elements)
+  int n_inner;       // columns: inner count
+  int nxh, ny;       // Z solve: spectrum geometry
+  double scale;      // Z solve: 1 / (nx ny nz)
+};
+struct GrpArgs {
+  const GrpDesc* d;
+  const int* tile_req;    // workgroup -> array
+  const int* tile_first;  // array -> its first workgroup
+};
+
+template <class T, bool R2C, int B, int NT, class RL, int W, bool TWL>
+__global__ __launch_bounds__(NT, W) void k_x_grp(XArgs a, GrpArgs g) {
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
+  const int r = g.tile_req[blockIdx.x];
+  const GrpDesc d = g.d[r];
+  a.in = d.in;
+  a.out = d.out;
+  a.nrows = d.nrows;
+  const long long pair0 = (long long)((int)blockIdx.x - g.tile_first[r]) * B;
+  const int nr = (int)(a.nrows - 2 * pair0 < 2 * B ? a.nrows - 2 * pair0 : 2 * B);
+  if constexpr (R2C) x_r2c_ct_tile<T, B, NT, RL, false>(lds, tw, a, pair0, pair0, nr);
+  else x_c2r_ct_tile<T, B, NT, RL, false>(lds, tw, a, pair0, pair0, nr);
+}
+
+template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL>
+__global__ __launch_bounds__(NT, W) void k_col_grp(ColArgs a, GrpArgs g) {
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
+  const int r = g.tile_req[blockIdx.x];
+  const GrpDesc d = g.d[r];
+  a.in = d.in; a.out = d.out;
+  a.n_inner = d.n_inner; a.n_outer = (int)d.nrows;
+  a.si_in = a.si_out = 1;
+  a.sl_in = a.sl_out = d.sl;
+  a.so_in = a.so_out = d.so;
+  const long long ncol = (long long)a.n_inner * a.n_outer;
+  const long long c0 = (long long)((int)blockIdx.x - g.tile_first[r]) * TX;
+  const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
+  const ColIO io = col_io<T, TX>(a, c0, nb);
+  run_fft_io<T, FWD, TX, NT, false, unsigned>(lds, tw, io, RL{});
+}
+
+template <class T, int TX, int NT, class RL, int W, bool TWL>
+__global__ __launch_bounds__(NT, W) void k_zs_grp(ColArgs a, GrpArgs g) {
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
+  const int r = g.tile_req[blockIdx.x];
+  const GrpDesc d = g.d[r];
+  a.in = d.in; a.out = d.out;
+  a.n_inner = d.n_inner; a.n_outer = 1;
+  a.si_in = a.si_out = 1;
+  a.sl_in = a.sl_out = d.sl;
+  a.so_in = a.so_out = 0;
+  const long long ncol = a.n_inner;
+  const long long c0 = (long long)((int)blockIdx.x - g.tile_first[r]) * TX;
+  const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
+  const ColIO io = col_io<T, TX>(a, c0, nb);
+  SolveCtx sc;
+  sc.nxh = d.nxh; sc.ny = d.ny; sc.nz = a.L.n; sc.scale = d.scale;
   run_fft_solve<T, TX, NT>(lds, tw, io, c0, sc, RL{});
 }
 
@@ -1267,6 +1344,10 @@ struct CtInfo {
   void (*xc2r)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*zsolve)(const ColArgs&, double, int, int, unsigned, size_t, hipStream_t);
   void (*prep)();
+  // grouped launches over many arrays of this length
+  void (*xgrp)(bool, const XArgs&, const GrpArgs&, unsigned, size_t, hipStream_t);
+  void (*cgrp)(bool, const ColArgs&, const GrpArgs&, unsigned, size_t, hipStream_t);
+  void (*zgrp)(const ColArgs&, const GrpArgs&, unsigned, size_t, hipStream_t);
   // fused X/Y passes (k_xy_ct): null when not built for this length
   void (*xy)(bool, const XArgs&, const FuseArgs&, unsigned, size_t, hipStream_t);
   int BF, NTF;          // row pairs per X item, threads
@@ -1354,6 +1435,17 @@ struct CtLen {
     allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, unsigned long long>);
     allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, unsigned>);
   }
+  static void xgrp(bool r2c, const XArgs& a, const GrpArgs& ga, unsigned g, size_t bytes, hipStream_t s) {
+    if (r2c) k_x_grp<real_t, true, BX, NTX, RL, WX, TWLX><<<g, NTX, bytes, s>>>(a, ga);
+    else k_x_grp<real_t, false, BX, NTX, RL, WX, TWLX><<<g, NTX, bytes, s>>>(a, ga);
+  }
+  static void cgrp(bool fwd, const ColArgs& a, const GrpArgs& ga, unsigned g, size_t bytes, hipStream_t s) {
+    if (fwd) k_col_grp<real_t, true, TX, NTC, RL, WC, TWLC><<<g, NTC, bytes, s>>>(a, ga);
+    else k_col_grp<real_t, false, TX, NTC, RL, WC, TWLC><<<g, NTC, bytes, s>>>(a, ga);
+  }
+  static void zgrp(const ColArgs& a, const GrpArgs& ga, unsigned g, size_t bytes, hipStream_t s) {
+    k_zs_grp<real_t, TX, NTC, RL, WC, TWLC><<<g, NTC, bytes, s>>>(a, ga);
+  }
   static void xy(bool fwd, const XArgs& a, const FuseArgs& f, unsigned g, size_t bytes, hipStream_t s) {
     if constexpr (FUSE) {
       if (fwd) k_xy_ct<real_t, true, RL, NTF, WF, TWLF, BF, TX><<<g, NTF, bytes, s>>>(a, f);
@@ -1367,6 +1459,11 @@ struct CtLen {
     }
     prep_dir<true>();
     prep_dir<false>();
+    allow_big_lds(k_x_grp<real_t, true, BX, NTX, RL, WX, TWLX>);
+    allow_big_lds(k_x_grp<real_t, false, BX, NTX, RL, WX, TWLX>);
+    allow_big_lds(k_col_grp<real_t, true, TX, NTC, RL, WC, TWLC>);
+    allow_big_lds(k_col_grp<real_t, false, TX, NTC, RL, WC, TWLC>);
+    allow_big_lds(k_zs_grp<real_t, TX, NTC, RL, WC, TWLC>);
     allow_big_lds(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX, true>);
     allow_big_lds(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX, false>);
     allow_big_lds(k_x_c2r_ct<real_t, BX, NTX, RL, WX, TWLX, true>);
@@ -1382,6 +1479,7 @@ struct CtLen {
     for (int i = 0; i < c.nr; i++) c.radix[i] = rs[i];
     c.twl_col = TWLC; c.twl_x = TWLX; c.twb = TWB;
     c.col = &col; c.colio = &colio; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve; c.prep = &prep;
+    c.xgrp = &xgrp; c.cgrp = &cgrp; c.zgrp = &zgrp;
     if constexpr (FUSE) {
       c.xy = &xy;
       c.BF = BF; c.NTF = NTF; c.DF = DF; c.twl_f = TWLF;
@@ -1749,6 +1847,178 @@ bool launch_zsolve(cplx_t* spec, int nxh, int ny, int nz, double scale, hipStrea
   ct->zsolve(a, scale, nxh, ny, (unsigned)a.tiles, bytes, st);
   check_hip(hipGetLastError(), "k_zsolve_ct launch");
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// grouped Poisson batches: the five passes of every solve of a batch as one
+// launch per pass and transform length
+// ---------------------------------------------------------------------------
+struct GrpStage {
+  int kind;  // 0 X r2c, 1 Y forward, 2 Z solve, 3 Y backward, 4 X c2r
+  const CtInfo* ct;
+  XArgs xa;
+  ColArgs ca;
+  GrpArgs ga;
+  unsigned nwg;
+  size_t bytes;
+};
+struct GroupBatch {
+  std::vector<GrpStage> st;
+  void* dmem = nullptr;
+};
+
+static bool grp_eligible(int nx, int ny, int nz, const CtInfo** cx, const CtInfo** cy, const CtInfo** cz) {
+  if (!g_use_ct || nx < 2 || (nx & 1)) return false;
+  *cx = find_ct(get_len(nx));
+  *cy = find_ct(get_len(ny));
+  *cz = find_ct(get_len(nz));
+  if (!*cx || !*cy || !*cz) return false;
+  const long long nxh = nx / 2 + 1, plane = nxh * ny;
+  // 32-bit tile offsets of every kernel (the single-array checks)
+  if ((2LL * (*cx)->BX + 1) * nxh * (long long)sizeof(cplx_t) >= (1LL << 32)) return false;
+  if ((plane * (nz + 2) + (*cz)->TX) * (long long)sizeof(cplx_t) >= (1LL << 32)) return false;
+  if (nxh * nz > 0x7fffffffLL || plane > 0x7fffffffLL) return false;
+  return true;
+}
+
+GroupBatch* group_batch_create(int n, const int* shapes, real_t* const* fields, void* const* works,
+                               std::vector<int>* singles) {
+  singles->clear();
+  struct Req {
+    int i, nx, ny, nz;
+    const CtInfo *cx, *cy, *cz;
+  };
+  std::vector<Req> rq;
+  for (int i = 0; i < n; i++) {
+    Req r{i, shapes[3 * i], shapes[3 * i + 1], shapes[3 * i + 2], nullptr, nullptr, nullptr};
+    if (grp_eligible(r.nx, r.ny, r.nz, &r.cx, &r.cy, &r.cz)) rq.push_back(r);
+    else singles->push_back(i);
+  }
+  GroupBatch* B = new GroupBatch;
+  // host images of every stage's descriptor and tile tables
+  struct HostStage {
+    GrpStage s;
+    std::vector<GrpDesc> d;
+    std::vector<int> treq, tfirst;
+  };
+  std::vector<HostStage> hs;
+  for (int kind = 0; kind < 5; kind++) {
+    std::map<int, std::vector<const Req*>> by_len;
+    for (const Req& r : rq) by_len[kind == 0 || kind == 4 ? r.nx : (kind == 2 ? r.nz : r.ny)].push_back(&r);
+    for (auto& kv : by_len) {
+      const int len = kv.first;
+      HostStage h;
+      std::memset(&h.s.xa, 0, sizeof(h.s.xa));
+      std::memset(&h.s.ca, 0, sizeof(h.s.ca));
+      h.s.kind = kind;
+      const Len1D& L = get_len(len);
+      const CtInfo* ct = find_ct(L);
+      h.s.ct = ct;
+      int tiles_total = 0;
+      for (const Req* r : kv.second) {
+        const int nxh = r->nx / 2 + 1;
+        GrpDesc d;
+        std::memset(&d, 0, sizeof(d));
+        long long tiles;
+        if (kind == 0 || kind == 4) {
+          d.in = kind == 0 ? (const void*)fields[r->i] : works[r->i];
+          d.out = kind == 0 ? works[r->i] : (void*)fields[r->i];
+          d.nrows = (long long)r->ny * r->nz;
+          tiles = ((d.nrows + 1) / 2 + ct->BX - 1) / ct->BX;
+        } else if (kind == 2) {
+          d.in = works[r->i];
+          d.out = works[r->i];
+          d.n_inner = nxh * r->ny;
+          d.sl = (long long)nxh * r->ny;
+          d.nxh = nxh;
+          d.ny = r->ny;
+          d.scale = 1.0 / ((double)r->nx * r->ny * r->nz);
+          tiles = ((long long)d.n_inner + ct->TX - 1) / ct->TX;
+        } else {
+          d.in = works[r->i];
+          d.out = works[r->i];
+          d.n_inner = nxh;
+          d.nrows = r->nz;
+          d.sl = nxh;
+          d.so = (long long)nxh * r->ny;
+          tiles = ((long long)nxh * r->nz + ct->TX - 1) / ct->TX;
+        }
+        if (tiles_total + tiles > 0x7fffffffLL) throw Error{"grouped batch too large", -1};
+        h.tfirst.push_back(tiles_total);
+        for (long long t = 0; t < tiles; t++) h.treq.push_back((int)h.d.size());
+        tiles_total += (int)tiles;
+        h.d.push_back(d);
+      }
+      h.s.nwg = (unsigned)tiles_total;
+      if (kind == 0 || kind == 4) {
+        XArgs& a = h.s.xa;
+        const int nxh = len / 2 + 1;
+        a.si_row = kind == 0 ? len : nxh;
+        a.so_row = kind == 0 ? nxh : len;
+        a.nxo = nxh;
+        a.dnxo = make_fdiv(nxh);
+        a.B = ct->BX;
+        a.L = L.d;
+        a.tw = L.tw;
+        h.s.bytes = ct_lds((size_t)(ct->BX + 1) * len * sizeof(cplx_t), ct->twl_x, *ct, &a.tw_lds, &a.tw_off);
+      } else {
+        ColArgs& a = h.s.ca;
+        a.kind = COL_C2C;
+        a.TX = ct->TX;
+        a.dTX = make_fdiv(ct->TX);
+        a.L = L.d;
+        a.tw = L.tw;
+        h.s.bytes = ct_lds((size_t)ct->TX * len * sizeof(cplx_t), ct->twl_col, *ct, &a.tw_lds, &a.tw_off);
+      }
+      if (h.s.bytes > kLdsMax) throw Error{"grouped pass does not fit in LDS", -1};
+      hs.push_back(std::move(h));
+    }
+  }
+  // one device allocation: per stage descriptors, tile -> array, first tiles
+  size_t total = 0;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  for (auto& h : hs) total += al(h.d.size() * sizeof(GrpDesc)) + al(h.treq.size() * 4) + al(h.tfirst.size() * 4);
+  if (total) check_hip(hipMalloc(&B->dmem, total), "hipMalloc(grouped batch)");
+  size_t off = 0;
+  for (auto& h : hs) {
+    char* base = (char*)B->dmem;
+    GrpDesc* dd = (GrpDesc*)(base + off);
+    off += al(h.d.size() * sizeof(GrpDesc));
+    int* tr = (int*)(base + off);
+    off += al(h.treq.size() * 4);
+    int* tf = (int*)(base + off);
+    off += al(h.tfirst.size() * 4);
+    check_hip(hipMemcpy(dd, h.d.data(), h.d.size() * sizeof(GrpDesc), hipMemcpyHostToDevice), "hipMemcpy");
+    check_hip(hipMemcpy(tr, h.treq.data(), h.treq.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    check_hip(hipMemcpy(tf, h.tfirst.data(), h.tfirst.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    h.s.ga.d = dd;
+    h.s.ga.tile_req = tr;
+    h.s.ga.tile_first = tf;
+    B->st.push_back(h.s);
+  }
+  return B;
+}
+
+void group_batch_launch(GroupBatch* B, hipStream_t st) {
+  for (const GrpStage& s : B->st) {
+    if (s.nwg == 0) continue;
+    switch (s.kind) {
+      case 0: s.ct->xgrp(true, s.xa, s.ga, s.nwg, s.bytes, st); break;
+      case 1: s.ct->cgrp(true, s.ca, s.ga, s.nwg, s.bytes, st); break;
+      case 2: s.ct->zgrp(s.ca, s.ga, s.nwg, s.bytes, st); break;
+      case 3: s.ct->cgrp(false, s.ca, s.ga, s.nwg, s.bytes, st); break;
+      default: s.ct->xgrp(false, s.xa, s.ga, s.nwg, s.bytes, st); break;
+    }
+    check_hip(hipGetLastError(), "grouped pass launch");
+  }
+}
+
+int group_batch_launches(const GroupBatch* B) { return (int)B->st.size(); }
+
+void group_batch_free(GroupBatch* B) {
+  if (!B) return;
+  if (B->dmem) (void)hipFree(B->dmem);
+  delete B;
 }
 
 // ---------------------------------------------------------------------------

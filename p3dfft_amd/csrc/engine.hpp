@@ -114,6 +114,17 @@ bool tile_order_alternate();
 // Returns false when nz has no specialised kernel (the caller composes it).
 bool launch_zsolve(cplx_t* spec, int nxh, int ny, int nz, double scale, hipStream_t st);
 
+// Grouped Poisson batches: the solves of n fields (shapes 3n ints; field i
+// real, works[i] its spectrum work area) whose lengths all have specialised
+// kernels run as one launch per pass and transform length; the indices of
+// the others are returned in `singles` (solved one by one by the caller).
+struct GroupBatch;
+GroupBatch* group_batch_create(int n, const int* shapes, real_t* const* fields, void* const* works,
+                               std::vector<int>* singles);
+void group_batch_launch(GroupBatch* B, hipStream_t st);
+int group_batch_launches(const GroupBatch* B);
+void group_batch_free(GroupBatch* B);
+
 // Fused X and Y passes (one persistent launch per direction, a work queue per
 // XCD, each plane's intermediate kept in that XCD's L2; opt-in, off by
 // default).  nullptr when the shape has no fused kernel (nx == ny specialised

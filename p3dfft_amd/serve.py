@@ -46,6 +46,14 @@ def _bind(L):
     L.p3dfft_poisson_graph_launch.restype = ctypes.c_int
     L.p3dfft_poisson_graph_destroy.argtypes = [vp]
     L.p3dfft_poisson_graph_destroy.restype = ctypes.c_int
+    L.p3dfft_poisson_batch_create.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.POINTER(vp)]
+    L.p3dfft_poisson_batch_create.restype = ctypes.c_int
+    L.p3dfft_poisson_batch_launch.argtypes = [vp, vp]
+    L.p3dfft_poisson_batch_launch.restype = ctypes.c_int
+    L.p3dfft_poisson_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.p3dfft_poisson_batch_info.restype = ctypes.c_int
+    L.p3dfft_poisson_batch_destroy.argtypes = [vp]
+    L.p3dfft_poisson_batch_destroy.restype = ctypes.c_int
     L._serve_bound = True
     return L
 
@@ -253,6 +261,10 @@ class PoissonServer:
         finally:
             L.p3dfft_amd_free(spec)
 
+    def batch(self, idx):
+        """A PoissonBatch (grouped launches) of the solves of requests `idx`."""
+        return PoissonBatch(self, idx)
+
     def graph(self, idx, nstreams=8):
         """A PoissonGraph of the solves of requests `idx` on their fields."""
         return PoissonGraph(self, idx, nstreams)
@@ -341,6 +353,61 @@ class PoissonGraph:
         L.p3dfft_amd_synchronize()
         if self.handle:
             L.p3dfft_poisson_graph_destroy(self.handle)
+            self.handle = None
+        if self.work_base:
+            L.p3dfft_amd_free(self.work_base)
+            self.work_base = None
+
+
+class PoissonBatch:
+    """A fixed batch of solves (requests `idx` of a PoissonServer) run by
+    p3dfft_poisson_batch_*: one launch per pass and transform length for every
+    solve whose lengths have specialised kernels, the rest one by one.  Work
+    areas (one per solve) are held for the batch's lifetime."""
+
+    def __init__(self, server, idx):
+        L = server.lib
+        self.lib = L
+        self.idx = list(idx)
+        n = len(self.idx)
+        works = [server.work[i] for i in self.idx]
+        offs = np.concatenate([[0], np.cumsum([(w + 255) & ~255 for w in works])]).astype(np.int64)
+        self.work_base = L.p3dfft_amd_malloc(int(offs[-1]))
+        if not self.work_base:
+            raise P3DFFTError("batch work allocation failed")
+        shapes = (ctypes.c_int * (3 * n))(*[v for i in self.idx for v in server.shapes[i]])
+        fields = (ctypes.c_void_p * n)(*[server.field_ptr(i) for i in self.idx])
+        wptr = (ctypes.c_void_p * n)(*[self.work_base + int(o) for o in offs[:-1]])
+        wbytes = (ctypes.c_size_t * n)(*works)
+        h = ctypes.c_void_p()
+        if L.p3dfft_poisson_batch_create(n, shapes, fields, wptr, wbytes, ctypes.byref(h)):
+            L.p3dfft_amd_free(self.work_base)
+            self.work_base = None
+            raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+        self.handle = h.value
+        gl, ss = ctypes.c_int(), ctypes.c_int()
+        L.p3dfft_poisson_batch_info(self.handle, ctypes.byref(gl), ctypes.byref(ss))
+        self.grouped_launches, self.single_solves = gl.value, ss.value
+
+    def launch(self, stream=None):
+        if self.lib.p3dfft_poisson_batch_launch(self.handle, stream):
+            raise P3DFFTError(self.lib.p3dfft_amd_last_error().decode())
+
+    def run(self, reps=1):
+        """`reps` back-to-back launches; returns seconds."""
+        L = self.lib
+        L.p3dfft_amd_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            self.launch()
+        L.p3dfft_amd_synchronize()
+        return time.perf_counter() - t0
+
+    def close(self):
+        L = self.lib
+        L.p3dfft_amd_synchronize()
+        if self.handle:
+            L.p3dfft_poisson_batch_destroy(self.handle)
             self.handle = None
         if self.work_base:
             L.p3dfft_amd_free(self.work_base)

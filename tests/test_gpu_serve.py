@@ -243,3 +243,39 @@ def test_graph_survives_plan_measurement():
             G.close()
         S.close()
     assert L.p3dfft_amd_reset_plans() == 0
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_grouped_batch_bit_identical_to_single_solves(precision):
+    """A batch run as grouped launches (one launch per pass and transform
+    length, each workgroup finding its field in a tile table) gives every
+    field bit for bit what p3dfft_poisson_solve gives it alone; shapes with
+    a generic or prime length run one by one inside the same launch call."""
+    shapes = [(32, 48, 64), (64, 32, 48), (48, 48, 48), (96, 64, 32), (32, 32, 32), (64, 96, 128),
+              (36, 32, 40), (17, 32, 48), (128, 64, 96), (48, 96, 64)]
+    S = serve.PoissonServer(shapes, precision=precision, slots=2)
+    B = None
+    try:
+        idx = list(range(len(shapes)))
+        S.fill()
+        rhs = [S.download(i) for i in idx]
+        for i in idx:
+            S.solve_alone(i)
+        want = [S.download(i) for i in idx]
+        B = S.batch(idx)
+        assert B.single_solves == 2  # (36, 32, 40) and (17, 32, 48)
+        # distinct lengths of the eight others: 5 in x, 4 in y, 5 in z
+        assert B.grouped_launches == 5 + 4 + 5 + 4 + 5
+        for rep in range(2):
+            S.fill()
+            B.launch()
+            S.lib.p3dfft_amd_synchronize()
+            for i in idx:
+                assert np.array_equal(S.download(i), want[i]), (rep, i, shapes[i])
+        tol = 2e-6 if precision == 4 else 1e-12
+        for i in idx:
+            assert rel(want[i], fp64_solve(rhs[i].astype(np.float64))) < tol * 10, shapes[i]
+    finally:
+        if B is not None:
+            B.close()
+        S.close()
