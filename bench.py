@@ -338,16 +338,29 @@ def batch_leg(n_req=256, slots=8, reps=20):
         Bt.run(reps=2)  # warm
         t_grp = min(Bt.run(reps=reps) for _ in range(3))
         Bt.close()
+        # one at a time: each solve synchronised before the next is issued
+        S.fill()
+        _, mk_seq, _ = S.run(idx, np.zeros(n_req), sequential=True)
     finally:
         if G is not None:
             G.close()
         S.close()
+    S8 = serve.PoissonServer(shapes, precision=8, slots=1)
+    try:
+        S8.fill()
+        S8.run(idx[:32], np.zeros(32), sequential=True)  # warm
+        _, mk_seq64, _ = S8.run(idx, np.zeros(n_req), sequential=True)
+    finally:
+        S8.close()
     eager, graph, grp = n_req * reps / t_eager, n_req * reps / t_graph, n_req * reps / t_grp
+    seq, seq64 = n_req / mk_seq, n_req / mk_seq64
     return {"requests": n_req, "branches": slots, "reps": reps,
             "eager_streams_solves_per_s": eager, "graph_solves_per_s": graph,
             "speedup_graph_vs_eager": graph / eager,
             "grouped_solves_per_s": grp, "grouped_launches_per_batch": launches,
             "speedup_grouped_vs_eager": grp / eager,
+            "one_at_a_time_solves_per_s": seq, "speedup_grouped_vs_one_at_a_time": grp / seq,
+            "one_at_a_time_fp64_solves_per_s": seq64, "speedup_grouped_vs_one_at_a_time_fp64": grp / seq64,
             "mean_points_per_request": float(np.mean([np.prod(s) for s in shapes]))}
 
 
