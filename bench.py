@@ -380,6 +380,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests of the served Poisson leg (0 = skip)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--batch-reps", type=int, default=20, help="repetitions of the batched leg's batch")
     ap.add_argument("--no-share", action="store_true", help="skip the share-mode (one rank of 2048^3) leg")
     ap.add_argument("--field", choices=["modes", "random"], default="modes",
                     help="modes: the 8-mode + noise field; random: N(0,1) (fast, for profiler runs)")
@@ -462,7 +463,7 @@ def main():
         log("serve %.1f solves/s (one-at-a-time %.1f)" % (line["serve"]["solves_per_s"],
                                                           line["serve"]["one_at_a_time_solves_per_s"]))
         log("batched leg (graph-captured vs eager streams)")
-        line["batched"] = batch_leg(slots=args.serve_slots)
+        line["batched"] = batch_leg(slots=args.serve_slots, reps=args.batch_reps)
         log("batched %.1f solves/s grouped, %.1f graph, %.1f eager" % (
             line["batched"]["grouped_solves_per_s"], line["batched"]["graph_solves_per_s"],
             line["batched"]["eager_streams_solves_per_s"]))
