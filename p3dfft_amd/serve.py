@@ -5,6 +5,9 @@ resident in HBM.  A request is admitted as soon as it has arrived, a stream
 slot is free and its work area fits the HBM budget (first-fit arena); its
 solve (r2c, -1/|k|^2 multiply, c2r: p3dfft_poisson_solve) is queued on the
 slot's stream, so admitted solves of different shapes overlap on the device.
+When the oldest waiting request does not fit, later arrived ones that do may
+go ahead of it a bounded number of times (no head-of-line blocking, no
+starvation).
 Completion is detected with HIP events, whose GPU timestamps give each
 request's latency (arrival to completion).  The same loop with one slot and a
 synchronise after every launch is the one-at-a-time baseline.
@@ -178,9 +181,15 @@ class PoissonServer:
         finally:
             self.arena.release(blk)
 
-    def run(self, order, arrivals, sequential=False):
+    def run(self, order, arrivals, sequential=False, lookahead=16, max_bypass=8):
         """Serve requests `order` (indices) arriving at `arrivals` seconds.
-        Returns per-request latency (s) and the makespan (s)."""
+        Admission is first-come first-served, except that when the oldest
+        waiting request does not fit the free HBM, up to `lookahead` later
+        arrived requests that do fit may be admitted ahead of it, at most
+        `max_bypass` times per head request (then admission waits for it), so a
+        large request neither blocks the small ones behind it nor starves.
+        Returns per-request latency (s), the makespan (s) and the peak work
+        bytes in use."""
         L = self.lib
         n = len(order)
         nslots = 1 if sequential else self.slots
@@ -190,6 +199,7 @@ class PoissonServer:
         free_slots = list(range(nslots))
         done = 0
         max_used = 0
+        bypassed = 0  # admissions ahead of the current head request
         L.p3dfft_amd_synchronize()
         L.p3dfft_amd_event_record(self.t0ev, None)
         L.p3dfft_amd_synchronize()
@@ -211,11 +221,28 @@ class PoissonServer:
             now = time.perf_counter() - t_start
             while pending and free_slots and arrivals[pending[0]] <= now:
                 k = pending[0]
+                blk = self.arena.alloc(self.work[order[k]])
+                if blk is not None:
+                    pending.popleft()
+                    bypassed = 0
+                else:
+                    # the head waits for memory; admit a later arrived request that fits
+                    if sequential or bypassed >= max_bypass:
+                        break
+                    k = None
+                    for j in range(1, min(lookahead, len(pending))):
+                        kj = pending[j]
+                        if arrivals[kj] > now:
+                            break
+                        blk = self.arena.alloc(self.work[order[kj]])
+                        if blk is not None:
+                            k = kj
+                            del pending[j]
+                            bypassed += 1
+                            break
+                    if k is None:
+                        break
                 i = order[k]
-                blk = self.arena.alloc(self.work[i])
-                if blk is None:
-                    break  # head of the queue waits for memory (FIFO admission)
-                pending.popleft()
                 slot = free_slots.pop()
                 st = self.streams[slot]
                 rc = L.p3dfft_poisson_solve(*self.shapes[i], self.field_ptr(i), self.arena.ptr(blk), self.work[i], st)

@@ -279,3 +279,32 @@ def test_grouped_batch_bit_identical_to_single_solves(precision):
         if B is not None:
             B.close()
         S.close()
+
+
+def test_small_requests_bypass_a_waiting_large_one():
+    """With HBM for one large solve only, a large request that must wait for
+    memory does not hold back the small requests behind it (some complete
+    before it), it still runs after a bounded number of bypasses, the budget
+    holds, and every result equals the unconstrained run."""
+    shapes = [(64, 64, 64), (160, 160, 160)] + [(32, 32, 32)] * 12
+    n = len(shapes)
+    S0 = serve.PoissonServer(shapes, precision=4, slots=4)
+    try:
+        S0.fill()
+        S0.run(list(range(n)), np.zeros(n))
+        ref = [S0.download(i) for i in range(n)]
+        work = list(S0.work)
+    finally:
+        S0.close()
+    budget = work[1] + 4096
+    S = serve.PoissonServer(shapes, precision=4, slots=4, work_budget=budget)
+    try:
+        S.fill()
+        lat, _, max_used = S.run(list(range(n)), np.zeros(n), max_bypass=8)
+        assert max_used <= budget
+        assert min(lat[2:]) < lat[1]  # a small request behind the large one finished first
+        assert sum(lat[2:] < lat[1]) <= 8 + 1  # bounded bypass (+ one already in flight)
+        for i in range(n):
+            assert np.array_equal(S.download(i), ref[i]), i
+    finally:
+        S.close()
