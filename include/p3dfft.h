@@ -187,6 +187,16 @@ int p3dfft_poisson_batch_create(int n, const int *shapes, p3dfft_real *const *fi
 int p3dfft_poisson_batch_launch(void *batch, void *stream);
 int p3dfft_poisson_batch_info(void *batch, int *grouped_launches, int *single_solves);
 int p3dfft_poisson_batch_destroy(void *batch);
+/* Batched 3D transforms of mixed shapes without a setup: direction 1 is r2c
+ * 'fft' of reals[i] (nx,ny,nz) into spectra[i] (nx/2+1,ny,nz), unnormalised
+ * (bit-identical to p3dfft_ftran_r2c_dev of that grid); -1 is c2r 'tff' of
+ * spectra[i] (overwritten) into reals[i].  One launch per pass and transform
+ * length; every length must have a specialised kernel (32, 48, 64, 96, 128,
+ * 160, 192, 256, 320, 384, 512, 1024, 2048). */
+int p3dfft_amd_batch_create(int direction, int n, const int *shapes, p3dfft_real *const *reals,
+                            p3dfft_real *const *spectra, void **batch);
+int p3dfft_amd_batch_launch(void *batch, void *stream);
+int p3dfft_amd_batch_destroy(void *batch);
 /* Capture the library's asynchronous _dev calls issued on `stream` (a created
  * stream, not NULL) between begin and end into a HIP graph; launch replays
  * them.  The stage timers pause during a capture; calls that allocate on

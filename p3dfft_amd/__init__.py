@@ -90,6 +90,12 @@ def load(precision=8):
     L.p3dfft_amd_set_x_into_out.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_fused_xy.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_plan_mode.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.POINTER(_vp)]
+    L.p3dfft_amd_batch_create.restype = ctypes.c_int
+    L.p3dfft_amd_batch_launch.argtypes = [_vp, _vp]
+    L.p3dfft_amd_batch_launch.restype = ctypes.c_int
+    L.p3dfft_amd_batch_destroy.argtypes = [_vp]
+    L.p3dfft_amd_batch_destroy.restype = ctypes.c_int
     L.p3dfft_amd_capture_begin.argtypes = [_vp]
     L.p3dfft_amd_capture_begin.restype = ctypes.c_int
     L.p3dfft_amd_capture_end.argtypes = [_vp, ctypes.POINTER(_vp)]

@@ -304,7 +304,7 @@ int p3dfft_poisson_batch_create(int n, const int* shapes, p3amd::real_t* const* 
     if (p3dfft_amd_prepare_length(shapes[i]) != 0) return -1;
   PoissonBatch* b = new PoissonBatch;
   try {
-    b->gb = group_batch_create(n, shapes, fields, works, &b->singles);
+    b->gb = group_batch_create(0, n, shapes, fields, works, &b->singles);
   } catch (const Error& e) {
     delete b;
     set_error(e.msg);
@@ -357,6 +357,63 @@ int p3dfft_poisson_batch_destroy(void* batch) {
   g_live_graphs--;
   return 0;
 }
+
+// Batched 3D transforms of mixed shapes (no setup needed): direction 1 is
+// r2c 'fft' of reals[i] (nx, ny, nz) into spectra[i] (nx/2+1, ny, nz),
+// unnormalised, as p3dfft_ftran_r2c_dev computes it; direction -1 is c2r
+// 'tff' of spectra[i] (overwritten) into reals[i].  One launch per pass and
+// transform length; every length must have a specialised kernel.
+int p3dfft_amd_batch_create(int direction, int n, const int* shapes, p3amd::real_t* const* reals,
+                            p3amd::real_t* const* spectra, void** batch_out) {
+  if (!batch_out || n <= 0 || !shapes || !reals || !spectra || (direction != 1 && direction != -1)) {
+    set_error("p3dfft_amd_batch_create: invalid arguments");
+    return -1;
+  }
+  *batch_out = nullptr;
+  for (int i = 0; i < n; i++) {
+    const int nx = shapes[3 * i], ny = shapes[3 * i + 1], nz = shapes[3 * i + 2];
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > max_len() || ny > max_len() || nz > max_len() || !reals[i] ||
+        !spectra[i]) {
+      set_error("p3dfft_amd_batch_create: invalid grid or array");
+      return -1;
+    }
+  }
+  for (int i = 0; i < 3 * n; i++)
+    if (p3dfft_amd_prepare_length(shapes[i]) != 0) return -1;
+  PoissonBatch* b = new PoissonBatch;
+  try {
+    b->gb = group_batch_create(direction == 1 ? 1 : 2, n, shapes, reals, (void* const*)spectra, &b->singles);
+  } catch (const Error& e) {
+    delete b;
+    set_error(e.msg);
+    return -1;
+  }
+  if (!b->singles.empty()) {
+    group_batch_free(b->gb);
+    delete b;
+    set_error("p3dfft_amd_batch_create: a length has no specialised kernel");
+    return -1;
+  }
+  g_live_graphs++;
+  *batch_out = b;
+  return 0;
+}
+
+int p3dfft_amd_batch_launch(void* batch, void* stream) {
+  if (!batch) {
+    set_error("p3dfft_amd_batch_launch: null batch");
+    return -1;
+  }
+  try {
+    group_batch_launch(((PoissonBatch*)batch)->gb, (hipStream_t)stream);
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return -1;
+  }
+  return 0;
+}
+
+int p3dfft_amd_batch_destroy(void* batch) { return p3dfft_poisson_batch_destroy(batch); }
 
 // Replay every captured solve on `stream` (asynchronous; the fields are
 // solved in place again, so the caller refreshes them between replays).

@@ -1881,7 +1881,7 @@ static bool grp_eligible(int nx, int ny, int nz, const CtInfo** cx, const CtInfo
   return true;
 }
 
-GroupBatch* group_batch_create(int n, const int* shapes, real_t* const* fields, void* const* works,
+GroupBatch* group_batch_create(int mode, int n, const int* shapes, real_t* const* fields, void* const* works,
                                std::vector<int>* singles) {
   singles->clear();
   struct Req {
@@ -1902,9 +1902,12 @@ GroupBatch* group_batch_create(int n, const int* shapes, real_t* const* fields, 
     std::vector<int> treq, tfirst;
   };
   std::vector<HostStage> hs;
-  for (int kind = 0; kind < 5; kind++) {
+  // passes: 0 X r2c, 1 Y forward, 2 Z solve, 3 Y backward, 4 X c2r, 5 Z forward, 6 Z backward
+  static const std::vector<int> kPasses[3] = {{0, 1, 2, 3, 4}, {0, 1, 5}, {6, 3, 4}};
+  for (int kind : kPasses[mode]) {
     std::map<int, std::vector<const Req*>> by_len;
-    for (const Req& r : rq) by_len[kind == 0 || kind == 4 ? r.nx : (kind == 2 ? r.nz : r.ny)].push_back(&r);
+    for (const Req& r : rq)
+      by_len[kind == 0 || kind == 4 ? r.nx : (kind == 1 || kind == 3 ? r.ny : r.nz)].push_back(&r);
     for (auto& kv : by_len) {
       const int len = kv.first;
       HostStage h;
@@ -1925,10 +1928,11 @@ GroupBatch* group_batch_create(int n, const int* shapes, real_t* const* fields, 
           d.out = kind == 0 ? works[r->i] : (void*)fields[r->i];
           d.nrows = (long long)r->ny * r->nz;
           tiles = ((d.nrows + 1) / 2 + ct->BX - 1) / ct->BX;
-        } else if (kind == 2) {
+        } else if (kind == 2 || kind == 5 || kind == 6) {
           d.in = works[r->i];
           d.out = works[r->i];
           d.n_inner = nxh * r->ny;
+          d.nrows = 1;
           d.sl = (long long)nxh * r->ny;
           d.nxh = nxh;
           d.ny = r->ny;
@@ -2007,6 +2011,8 @@ void group_batch_launch(GroupBatch* B, hipStream_t st) {
       case 1: s.ct->cgrp(true, s.ca, s.ga, s.nwg, s.bytes, st); break;
       case 2: s.ct->zgrp(s.ca, s.ga, s.nwg, s.bytes, st); break;
       case 3: s.ct->cgrp(false, s.ca, s.ga, s.nwg, s.bytes, st); break;
+      case 5: s.ct->cgrp(true, s.ca, s.ga, s.nwg, s.bytes, st); break;
+      case 6: s.ct->cgrp(false, s.ca, s.ga, s.nwg, s.bytes, st); break;
       default: s.ct->xgrp(false, s.xa, s.ga, s.nwg, s.bytes, st); break;
     }
     check_hip(hipGetLastError(), "grouped pass launch");

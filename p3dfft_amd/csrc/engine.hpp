@@ -114,12 +114,14 @@ bool tile_order_alternate();
 // Returns false when nz has no specialised kernel (the caller composes it).
 bool launch_zsolve(cplx_t* spec, int nxh, int ny, int nz, double scale, hipStream_t st);
 
-// Grouped Poisson batches: the solves of n fields (shapes 3n ints; field i
-// real, works[i] its spectrum work area) whose lengths all have specialised
-// kernels run as one launch per pass and transform length; the indices of
-// the others are returned in `singles` (solved one by one by the caller).
+// Grouped batches over n fields (shapes 3n ints; field i real, works[i] its
+// (nx/2+1, ny, nz) complex spectrum) whose lengths all have specialised
+// kernels: one launch per pass and transform length.  mode 0: Poisson solves
+// in place on the fields (spectrum as work area); 1: forward r2c 'fft' of the
+// fields into the spectra; 2: backward c2r 'tff' of the spectra (destroyed)
+// into the fields.  Indices of the other fields are returned in `singles`.
 struct GroupBatch;
-GroupBatch* group_batch_create(int n, const int* shapes, real_t* const* fields, void* const* works,
+GroupBatch* group_batch_create(int mode, int n, const int* shapes, real_t* const* fields, void* const* works,
                                std::vector<int>* singles);
 void group_batch_launch(GroupBatch* B, hipStream_t st);
 int group_batch_launches(const GroupBatch* B);

@@ -709,3 +709,56 @@ def test_captured_transforms_replay_bit_exact(precision, shape):
             for b in (fin, spec, out, other, ospec):
                 b.free()
             L.p3dfft_amd_stream_destroy(st)
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_batched_transforms_of_mixed_shapes_bit_identical(precision):
+    """p3dfft_amd_batch_*: forward and backward transforms of several grids of
+    different shapes as one launch per pass and transform length give, grid
+    by grid, bit for bit the spectra and round trips of p3dfft_setup +
+    ftran_r2c / btran_c2r of that grid; a length without a specialised kernel
+    is refused."""
+    import ctypes
+    shapes = [(64, 48, 32), (32, 64, 96), (48, 48, 48), (128, 32, 64), (96, 96, 32)]
+    rng = np.random.default_rng(61)
+    fields = [rng.standard_normal((nz, ny, nx)) for (nx, ny, nz) in shapes]
+    want_s, want_b = [], []
+    for (nx, ny, nz), f in zip(shapes, fields):
+        with lib_setup(precision, nx, ny, nz) as P:
+            s = P.ftran_r2c(f, "fft")
+            want_s.append(s)
+            want_b.append(P.btran_c2r(s.copy(), "tff"))
+    L = p3dfft_amd.load(precision)
+    P = p3dfft_amd.P3DFFT(precision)
+    rdt, cdt = P.rdtype, P.cdtype
+    n = len(shapes)
+    reals = [p3dfft_amd.DeviceBuffer(L, nx * ny * nz * np.dtype(rdt).itemsize) for (nx, ny, nz) in shapes]
+    specs = [p3dfft_amd.DeviceBuffer(L, (nx // 2 + 1) * ny * nz * np.dtype(cdt).itemsize) for (nx, ny, nz) in shapes]
+    sh = (ctypes.c_int * (3 * n))(*[v for s in shapes for v in s])
+    rp = (ctypes.c_void_p * n)(*[b.ptr for b in reals])
+    sp = (ctypes.c_void_p * n)(*[b.ptr for b in specs])
+    fwd, bwd = ctypes.c_void_p(), ctypes.c_void_p()
+    try:
+        assert L.p3dfft_amd_batch_create(1, n, sh, rp, sp, ctypes.byref(fwd)) == 0
+        assert L.p3dfft_amd_batch_create(-1, n, sh, rp, sp, ctypes.byref(bwd)) == 0
+        for b, f in zip(reals, fields):
+            b.upload(f.astype(rdt))
+        assert L.p3dfft_amd_batch_launch(fwd, None) == 0
+        assert L.p3dfft_amd_synchronize() == 0
+        for i, (nx, ny, nz) in enumerate(shapes):
+            assert np.array_equal(specs[i].download((nz, ny, nx // 2 + 1), cdt), want_s[i]), shapes[i]
+        assert L.p3dfft_amd_batch_launch(bwd, None) == 0
+        assert L.p3dfft_amd_synchronize() == 0
+        for i, (nx, ny, nz) in enumerate(shapes):
+            assert np.array_equal(reals[i].download((nz, ny, nx), rdt), want_b[i]), shapes[i]
+        bad = (ctypes.c_int * 3)(36, 32, 32)
+        h = ctypes.c_void_p()
+        assert L.p3dfft_amd_batch_create(1, 1, bad, rp, sp, ctypes.byref(h)) == -1
+        assert b"specialised" in L.p3dfft_amd_last_error()
+        L.p3dfft_amd_clear_error()
+    finally:
+        for h in (fwd, bwd):
+            if h.value:
+                L.p3dfft_amd_batch_destroy(h)
+        for b in reals + specs:
+            b.free()
