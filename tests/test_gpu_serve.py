@@ -308,3 +308,29 @@ def test_small_requests_bypass_a_waiting_large_one():
             assert np.array_equal(S.download(i), ref[i]), i
     finally:
         S.close()
+
+
+def test_grouped_batch_on_the_served_size_mix():
+    """40 requests drawn like the served workload (axes from {32, ..., 384},
+    every specialised length, radix 3 and 5 included): the grouped batch
+    equals the one-by-one solves bit for bit."""
+    shapes = serve.make_requests(40, seed=71)
+    S = serve.PoissonServer(shapes, precision=4, slots=2)
+    B = None
+    try:
+        idx = list(range(len(shapes)))
+        S.fill()
+        for i in idx:
+            S.solve_alone(i)
+        want = [S.download(i) for i in idx]
+        B = S.batch(idx)
+        assert B.single_solves == 0
+        S.fill()
+        B.launch()
+        S.lib.p3dfft_amd_synchronize()
+        for i in idx:
+            assert np.array_equal(S.download(i), want[i]), (i, shapes[i])
+    finally:
+        if B is not None:
+            B.close()
+        S.close()
