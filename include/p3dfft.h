@@ -205,6 +205,19 @@ int p3dfft_amd_capture_begin(void *stream);
 int p3dfft_amd_capture_end(void *stream, void **graph);
 int p3dfft_amd_graph_launch(void *graph, void *stream);
 int p3dfft_amd_graph_destroy(void *graph);
+/* The continuous-batching loop of served solves: request k solves field
+ * order[k] (shapes 3 ints per field, fields, work_bytes per field) and arrives
+ * arrivals[k] seconds after the start; admitted once arrived, a stream slot
+ * (of `slots`) is free and its work area fits a first-fit arena over the HBM
+ * region [arena, arena+arena_bytes); when the oldest waiting request does not
+ * fit, up to `lookahead` later ones that do may go first, at most
+ * `max_bypass` times per waiting request.  sequential != 0: one request at a
+ * time.  Outputs per-request latency (s, from GPU event timestamps), the
+ * makespan (s) and the peak work bytes. */
+int p3dfft_serve_run(int n, const int *shapes, p3dfft_real *const *fields, const size_t *work_bytes,
+                     const int *order, const double *arrivals, int slots, void *arena, size_t arena_bytes,
+                     int sequential, int lookahead, int max_bypass, double *latency, double *makespan,
+                     size_t *peak);
 /* build the plan and twiddle table of a transform length ahead of time */
 int p3dfft_amd_prepare_length(int n);
 /* i.i.d. N(0,1) values from a counter-based generator keyed by seed */

@@ -1,4 +1,5 @@
-"""Continuous-batching service of spectral Poisson solves on one MI355X.
+"""Continuous-batching service of spectral Poisson solves on one MI355X
+(the loop itself is native: p3dfft_serve_run in csrc/serve.cpp).
 
 Requests (mixed grid shapes) arrive over time with their right-hand sides
 resident in HBM.  A request is admitted as soon as it has arrived, a stream
@@ -14,7 +15,6 @@ synchronise after every launch is the one-at-a-time baseline.
 """
 import ctypes
 import time
-from collections import deque
 
 import numpy as np
 
@@ -57,6 +57,10 @@ def _bind(L):
     L.p3dfft_poisson_batch_info.restype = ctypes.c_int
     L.p3dfft_poisson_batch_destroy.argtypes = [vp]
     L.p3dfft_poisson_batch_destroy.restype = ctypes.c_int
+    L.p3dfft_serve_run.argtypes = [ctypes.c_int, vp, vp, vp, vp, vp, ctypes.c_int, vp, ctypes.c_size_t,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_size_t)]
+    L.p3dfft_serve_run.restype = ctypes.c_int
     L._serve_bound = True
     return L
 
@@ -182,82 +186,36 @@ class PoissonServer:
             self.arena.release(blk)
 
     def run(self, order, arrivals, sequential=False, lookahead=16, max_bypass=8):
-        """Serve requests `order` (indices) arriving at `arrivals` seconds.
-        Admission is first-come first-served, except that when the oldest
-        waiting request does not fit the free HBM, up to `lookahead` later
-        arrived requests that do fit may be admitted ahead of it, at most
-        `max_bypass` times per head request (then admission waits for it), so a
-        large request neither blocks the small ones behind it nor starves.
+        """Serve requests `order` (indices) arriving at `arrivals` seconds,
+        by the native loop p3dfft_serve_run over this server's fields, work
+        arena and `slots` streams.  Admission is first-come first-served,
+        except that when the oldest waiting request does not fit the free HBM,
+        up to `lookahead` later arrived requests that do fit may be admitted
+        ahead of it, at most `max_bypass` times per waiting request, so a large
+        request neither blocks the small ones behind it nor starves.
+        sequential: one request at a time, synchronised after each launch.
         Returns per-request latency (s), the makespan (s) and the peak work
         bytes in use."""
         L = self.lib
         n = len(order)
-        nslots = 1 if sequential else self.slots
+        if self.arena.used:
+            raise P3DFFTError("the work arena must be free when serving")
+        nf = len(self.shapes)
+        shapes = (ctypes.c_int * (3 * nf))(*[v for sh in self.shapes for v in sh])
+        fields = (ctypes.c_void_p * nf)(*[self.field_ptr(i) for i in range(nf)])
+        wb = (ctypes.c_size_t * nf)(*self.work)
+        od = (ctypes.c_int * n)(*[int(i) for i in order])
+        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
         lat = np.zeros(n)
-        pending = deque(range(n))
-        busy = {}  # slot -> (k, blk)
-        free_slots = list(range(nslots))
-        done = 0
-        max_used = 0
-        bypassed = 0  # admissions ahead of the current head request
-        L.p3dfft_amd_synchronize()
-        L.p3dfft_amd_event_record(self.t0ev, None)
-        L.p3dfft_amd_synchronize()
-        t_start = time.perf_counter()
-        last_done_ms = 0.0
-        while done < n:
-            progress = False
-            for slot in list(busy):
-                k, blk = busy[slot]
-                if L.p3dfft_amd_event_query(self.events[slot]):
-                    t_ms = L.p3dfft_amd_event_elapsed_ms(self.t0ev, self.events[slot])
-                    lat[k] = max(t_ms * 1e-3 - arrivals[k], 0.0)
-                    last_done_ms = max(last_done_ms, t_ms)
-                    self.arena.release(blk)
-                    del busy[slot]
-                    free_slots.append(slot)
-                    done += 1
-                    progress = True
-            now = time.perf_counter() - t_start
-            while pending and free_slots and arrivals[pending[0]] <= now:
-                k = pending[0]
-                blk = self.arena.alloc(self.work[order[k]])
-                if blk is not None:
-                    pending.popleft()
-                    bypassed = 0
-                else:
-                    # the head waits for memory; admit a later arrived request that fits
-                    if sequential or bypassed >= max_bypass:
-                        break
-                    k = None
-                    for j in range(1, min(lookahead, len(pending))):
-                        kj = pending[j]
-                        if arrivals[kj] > now:
-                            break
-                        blk = self.arena.alloc(self.work[order[kj]])
-                        if blk is not None:
-                            k = kj
-                            del pending[j]
-                            bypassed += 1
-                            break
-                    if k is None:
-                        break
-                i = order[k]
-                slot = free_slots.pop()
-                st = self.streams[slot]
-                rc = L.p3dfft_poisson_solve(*self.shapes[i], self.field_ptr(i), self.arena.ptr(blk), self.work[i], st)
-                if rc:
-                    raise P3DFFTError(L.p3dfft_amd_last_error().decode())
-                L.p3dfft_amd_event_record(self.events[slot], st)
-                busy[slot] = (k, blk)
-                max_used = max(max_used, self.arena.used)
-                progress = True
-                if sequential:
-                    L.p3dfft_amd_stream_synchronize(st)
-            if not progress:
-                time.sleep(2e-5)
-        makespan = last_done_ms * 1e-3 - float(arrivals[0])
-        return lat, makespan, max_used
+        mk = ctypes.c_double()
+        pk = ctypes.c_size_t()
+        rc = L.p3dfft_serve_run(n, shapes, fields, wb, od, arr.ctypes.data, self.slots, self.arena.base,
+                                self.arena.size, 1 if sequential else 0, int(lookahead), int(max_bypass),
+                                lat.ctypes.data, ctypes.byref(mk), ctypes.byref(pk))
+        if rc:
+            raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+        self.arena.peak = max(self.arena.peak, int(pk.value))
+        return lat, mk.value, int(pk.value)
 
     def run_api(self, order):
         """One request at a time through the reference's entry points, as a

@@ -121,3 +121,25 @@ def test_plan_report_empty_before_any_measurement():
     buf = ctypes.create_string_buffer(64)
     assert L.p3dfft_amd_plan_report(buf, 64) == 0
     assert buf.value == b""
+
+
+def test_serve_loop_rejects_inadmissible_requests_without_gpu():
+    """The native serving loop refuses a request whose work area can never
+    fit the arena (it would wait forever) before any device work."""
+    import ctypes
+    from p3dfft_amd import serve
+    L = serve._bind(p3dfft_amd.load(4))
+    lat = (ctypes.c_double * 1)()
+    mk = ctypes.c_double()
+    pk = ctypes.c_size_t()
+    assert L.p3dfft_serve_run(0, None, None, None, None, None, 1, None, 0, 0, 16, 8, lat, ctypes.byref(mk),
+                              ctypes.byref(pk)) == -1
+    shapes = (ctypes.c_int * 3)(32, 32, 32)
+    fields = (ctypes.c_void_p * 1)(16)
+    wb = (ctypes.c_size_t * 1)(1 << 20)
+    order = (ctypes.c_int * 1)(0)
+    arr = (ctypes.c_double * 1)(0.0)
+    assert L.p3dfft_serve_run(1, shapes, fields, wb, order, arr, 1, 4096, 1 << 19, 0, 16, 8, lat,
+                              ctypes.byref(mk), ctypes.byref(pk)) == -1
+    assert b"exceeds the arena" in L.p3dfft_amd_last_error()
+    L.p3dfft_amd_clear_error()
