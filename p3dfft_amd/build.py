@@ -80,6 +80,11 @@ def build_fortran():
         _run([FLANG, "-cpp", "-O2"] + defs + ["-I", md, os.path.join(HERE, "fortran", "examples", "driver_sine.f90"),
               "-o", exe, "-L" + LIB, "-l" + os.path.splitext(lib)[0][3:], "-l" + base, "-Wl,-rpath,$ORIGIN"])
         outs.append(exe)
+        # the utility procedures' check (examples/driver_utils.f90)
+        exe = os.path.join(LIB, "driver_utils_f.x" if not defs else "driver_utils_f_sp.x")
+        _run([FLANG, "-cpp", "-O2"] + defs + ["-I", md, os.path.join(HERE, "fortran", "examples", "driver_utils.f90"),
+              "-o", exe, "-L" + LIB, "-l" + os.path.splitext(lib)[0][3:], "-l" + base, "-Wl,-rpath,$ORIGIN"])
+        outs.append(exe)
     return outs
 
 

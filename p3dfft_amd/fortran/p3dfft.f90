@@ -28,7 +28,12 @@ module p3dfft
 
   public :: p3dfft_setup, p3dfft_get_dims, p3dfft_ftran_r2c, p3dfft_btran_c2r, &
             p3dfft_cheby, p3dfft_ftran_r2c_many, p3dfft_btran_c2r_many, p3dfft_cheby_many, &
-            p3dfft_ftran_r2c_1d, p3dfft_clean, get_timers, set_timers, p3dfft_set_mpi_types
+            p3dfft_ftran_r2c_1d, p3dfft_clean, get_timers, set_timers, p3dfft_set_mpi_types, &
+            p3dfft_get_mpi_info, print_buf, print_buf_real, rtran_x2y, rtran_y2x, rtran_x2z, rtran_z2x
+
+  ! the communicator handed to p3dfft_setup (one GPU is task 0 of 1)
+  integer, save :: setup_comm = -1
+  logical, save :: setup_done = .false.
 
   interface
     subroutine c_p3dfft_setup(dims, nx, ny, nz, comm, nxc, nyc, nzc, ow, memsize) &
@@ -106,6 +111,8 @@ contains
     call c_p3dfft_setup(d, cnx, cny, cnz, comm, cx, cy, cz, ow, ms)
     if (present(memsize)) memsize = ms
     timers = 0.0_r8
+    setup_comm = mpi_comm_in
+    setup_done = .true.
   end subroutine
 
   ! build/module.F90:225-273
@@ -200,6 +207,97 @@ contains
   ! build/module.F90:309
   subroutine p3dfft_clean()
     call c_p3dfft_clean()
+    setup_done = .false.
+  end subroutine
+
+  ! build/module.F90:280-297 on one GPU: task 0 of 1, the setup's communicator
+  subroutine p3dfft_get_mpi_info(mpi_taskid, mpi_tasks, mpi_comm)
+    integer, intent(out) :: mpi_taskid, mpi_tasks, mpi_comm
+    if (.not. setup_done) then
+      print *, 'P3DFFT error: call setup before other routines'
+      return
+    end if
+    mpi_taskid = 0
+    mpi_tasks = 1
+    mpi_comm = setup_comm
+  end subroutine
+
+  ! build/module.F90:747-781: every entry above 5e-7 in magnitude, with its
+  ! indices, as task 0
+  subroutine print_buf(A, lx, ly, lz)
+    integer, intent(in) :: lx, ly, lz
+    complex(p3dfft_type), intent(in) :: A(lx, ly, lz)
+    integer :: i, j, k
+    do k = 1, lz
+      do j = 1, ly
+        do i = 1, lx
+          if (abs(A(i, j, k)) > 0.0000005) print *, 0, ': (', i, j, k, ') =', A(i, j, k)
+        end do
+      end do
+    end do
+  end subroutine
+
+  subroutine print_buf_real(A, lx, ly, lz)
+    integer, intent(in) :: lx, ly, lz
+    real(p3dfft_type), intent(in) :: A(lx, ly, lz)
+    integer :: i, j, k
+    do k = 1, lz
+      do j = 1, ly
+        do i = 1, lx
+          if (abs(A(i, j, k)) > 0.0000005) print *, 0, ': (', i, j, k, ') =', A(i, j, k)
+        end do
+      end do
+    end do
+  end subroutine
+
+  ! build/module.F90:1061-1361, the real-data pencil transposes.  On one GPU
+  ! the X, Y and Z pencils of real data are all the whole (nx, ny, nz) grid,
+  ! so each is a copy with the grid's bounds; the exchange buffers and the
+  ! exchange time t are untouched.
+  subroutine rtran_x2y(source, dest, rbuf1, rbuf2, dstart, dend, dsize, t)
+    real(p3dfft_type), intent(in) :: source(*)
+    real(p3dfft_type), intent(out) :: dest(*)
+    real(p3dfft_type), intent(inout) :: rbuf1(*), rbuf2(*)
+    integer, intent(out) :: dstart(3), dend(3), dsize(3)
+    real(r8) :: t
+    call rtran_whole(source, dest, dstart, dend, dsize)
+  end subroutine
+
+  subroutine rtran_y2x(source, dest, rbuf1, rbuf2, dstart, dend, dsize, t)
+    real(p3dfft_type), intent(in) :: source(*)
+    real(p3dfft_type), intent(out) :: dest(*)
+    real(p3dfft_type), intent(inout) :: rbuf1(*), rbuf2(*)
+    integer, intent(out) :: dstart(3), dend(3), dsize(3)
+    real(r8) :: t
+    call rtran_whole(source, dest, dstart, dend, dsize)
+  end subroutine
+
+  subroutine rtran_x2z(source, dest, rbuf1, rbuf2, dstart, dend, dsize, t)
+    real(p3dfft_type), intent(in) :: source(*)
+    real(p3dfft_type), intent(out) :: dest(*)
+    real(p3dfft_type), intent(inout) :: rbuf1(*), rbuf2(*)
+    integer, intent(out) :: dstart(3), dend(3), dsize(3)
+    real(r8) :: t
+    call rtran_whole(source, dest, dstart, dend, dsize)
+  end subroutine
+
+  subroutine rtran_z2x(source, dest, rbuf1, rbuf2, dstart, dend, dsize, t)
+    real(p3dfft_type), intent(in) :: source(*)
+    real(p3dfft_type), intent(out) :: dest(*)
+    real(p3dfft_type), intent(inout) :: rbuf1(*), rbuf2(*)
+    integer, intent(out) :: dstart(3), dend(3), dsize(3)
+    real(r8) :: t
+    call rtran_whole(source, dest, dstart, dend, dsize)
+  end subroutine
+
+  subroutine rtran_whole(source, dest, dstart, dend, dsize)
+    real(p3dfft_type), intent(in) :: source(*)
+    real(p3dfft_type), intent(out) :: dest(*)
+    integer, intent(out) :: dstart(3), dend(3), dsize(3)
+    integer(i8) :: n
+    call p3dfft_get_dims(dstart, dend, dsize, 1)
+    n = int(dsize(1), i8) * dsize(2) * dsize(3)
+    dest(1:n) = source(1:n)
   end subroutine
 
   ! build/module.F90:730-744

@@ -20,3 +20,23 @@ def test_fortran_driver_sine(exe):
     print(r.stdout)
     assert r.returncode == 0
     assert "Results are correct" in r.stdout
+
+
+@pytest.mark.parametrize("exe", ["driver_utils_f.x", "driver_utils_f_sp.x"])
+def test_fortran_module_utilities(exe):
+    """p3dfft_get_mpi_info, rtran_x2y/y2x/x2z/z2x and print_buf of module
+    p3dfft (build/module.F90:178-186 publics) on one GPU: task 0 of 1 with the
+    setup's communicator, the transposes copy the whole grid with its bounds,
+    print_buf lists exactly the non-zero modes."""
+    path = os.path.join(LIB, exe)
+    if not os.path.exists(path):
+        pytest.fail("Fortran utilities driver not built: %s" % path)
+    r = subprocess.run([path], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0
+    assert "Utilities are correct" in r.stdout
+    lines = [l for l in r.stdout.splitlines() if ": (" in l]
+    assert len(lines) == 2
+    # loop order: k outermost, then j, then i
+    assert [int(v) for v in lines[0].split("(")[1].split(")")[0].split()] == [5, 1, 1]
+    assert [int(v) for v in lines[1].split("(")[1].split(")")[0].split()] == [2, 3, 4]
