@@ -7,15 +7,22 @@ the timed region).  The headline is the FP32 build (FP32 bulk, compensated
 hi+lo FP32 twiddles generated in FP64); the plain FP64 build is timed beside
 it on the same field.
 
-Prints ONE JSON line.  Fields beyond the driver's contract:
+Prints ONE JSON line.  Besides metric, value, unit, steps and timing:
   roofline       dominant kernel: algorithmic bytes per launch / HIP-event
                  duration per launch, against the 8 TB/s HBM peak
   path           whole fwd+bwd against the compulsory-bytes roofline
                  2B = 2 (N_r s + N_c 2s) and the measured copy-kernel peak
+  kernels        per-stage durations from the library's HIP-event timers
   fp64_*         the plain FP64 GPU path and the FP32 speed-up over it
   rel_l2_*       FP32 spectrum / round trip vs the FP64 path
+  power_spectrum the drivers' shell sums of the 512^3 FP32 spectrum
+  share          one rank's stages of 2048^3 on a 16 x 16 grid
+  serve          served mixed-size Poisson solves: throughput, p50/p99,
+                 one-at-a-time baselines
+  batched        a fixed batch of small solves: eager streams, one HIP
+                 graph, grouped launches, one at a time
   cpu_baseline   the FP64 CPU oracle (a restatement of the reference's CPU
-                 algorithm) on a bounded sample, host cores of this box
+                 algorithm) on a bounded sample, host cores of this machine
 """
 import argparse
 import json
