@@ -16,6 +16,7 @@ Prints ONE JSON line.  Besides metric, value, unit, steps and timing:
   fp64_*         the plain FP64 GPU path and the FP32 speed-up over it
   rel_l2_*       FP32 spectrum / round trip vs the FP64 path
   power_spectrum the drivers' shell sums of the 512^3 FP32 spectrum
+  c1_64cubed     64^3 FP64 fwd+bwd, eager and replayed from a HIP graph
   share          one rank's stages of 2048^3 on a 16 x 16 grid
   serve          served mixed-size Poisson solves: throughput, p50/p99,
                  one-at-a-time baselines
@@ -321,6 +322,57 @@ def serve_leg(n_req, slots, load_frac=0.7):
             "mean_points_per_request": float(np.mean([np.prod(s) for s in shapes]))}
 
 
+def small_leg(n=64, reps=200):
+    """The CPU-runnable 64^3 FP64 case on the GPU: fwd+bwd through the device
+    entry points issued eagerly, and the same pair captured once into a HIP
+    graph (p3dfft_amd_capture_*) and replayed."""
+    import ctypes
+    P = p3dfft_amd.P3DFFT(8)
+    P.setup((1, 1), n, n, n)
+    L = P.lib
+    st = L.p3dfft_amd_stream_create()
+    fin, spec, out = P.alloc(P.real_bytes()), P.alloc(P.spec_bytes()), P.alloc(P.real_bytes())
+    g = ctypes.c_void_p()
+    try:
+        fin.upload(np.random.default_rng(1).random((n, n, n)))
+        e0, e1 = L.p3dfft_amd_event_create(), L.p3dfft_amd_event_create()
+
+        def timed(issue):
+            for _ in range(10):
+                issue()
+            L.p3dfft_amd_stream_synchronize(st)
+            L.p3dfft_amd_event_record(e0, st)
+            for _ in range(reps):
+                issue()
+            L.p3dfft_amd_event_record(e1, st)
+            L.p3dfft_amd_stream_synchronize(st)
+            return L.p3dfft_amd_event_elapsed_ms(e0, e1) / reps
+
+        def eager():
+            P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft", st)
+            P.btran_c2r_dev(spec.ptr, out.ptr, "tff", st)
+
+        P.lib.p3dfft_amd_set_timing(0)
+        t_eager = timed(eager)
+        if L.p3dfft_amd_capture_begin(st):
+            raise RuntimeError("capture failed")
+        eager()
+        if L.p3dfft_amd_capture_end(st, ctypes.byref(g)):
+            raise RuntimeError("capture failed")
+        t_graph = timed(lambda: L.p3dfft_amd_graph_launch(g, st))
+        L.p3dfft_amd_event_destroy(e0)
+        L.p3dfft_amd_event_destroy(e1)
+    finally:
+        if g.value:
+            L.p3dfft_amd_graph_destroy(g)
+        fin.free(); spec.free(); out.free()
+        L.p3dfft_amd_stream_destroy(st)
+        P.lib.p3dfft_amd_set_timing(1)
+        P.clean()
+    return {"grid": [n, n, n], "precision": "f64", "eager_ms_per_fwd_bwd": t_eager,
+            "graph_ms_per_fwd_bwd": t_graph, "reps": reps}
+
+
 def batch_leg(n_req=256, slots=8, reps=20):
     """A fixed batch of small mixed-size solves (axes from {32,...,128}, FP32)
     run `reps` times: issued eagerly on `slots` streams, captured once into a
@@ -460,6 +512,8 @@ def main():
         line["rel_l2_spectrum_fp32_vs_fp64"] = float(np.linalg.norm(r32["spec"] - ref) / np.linalg.norm(ref))
         b64 = r64["back"].astype(np.float64)
         line["rel_l2_roundtrip_fp32_vs_fp64"] = float(np.linalg.norm(r32["back"] - b64) / np.linalg.norm(b64))
+    log("64^3 FP64 leg (eager and graph-captured)")
+    line["c1_64cubed"] = small_leg()
     if not args.no_share:
         log("share-mode leg (one rank of 2048^3 on 16x16)")
         line["share"] = share_leg()
