@@ -1285,16 +1285,17 @@ static void dispatch_nt(int nt, const A& a, unsigned g, size_t b, hipStream_t s)
 // ---- compile-time specialised lengths ----------------------------------
 // Tuning constants of the specialised kernels (compile-time): complex values
 // per thread, bytes of adjacent columns per row of a column tile, and the
-// largest column workgroup (FP32: 512 threads of 16 values measured 3% faster
-// over a 512^3 fwd+bwd than 256 threads of 32).
+// largest column workgroup.  The FP32 column passes are bound by the memory
+// requests in flight per CU: over a 512^3 fwd+bwd, 256 threads of 32 values
+// took 1.41 ms, 512 of 16 1.36 ms, 1024 of 8 1.32 ms.
 #ifndef P3DFFT_CT_EPT
-#define P3DFFT_CT_EPT 16
+#define P3DFFT_CT_EPT (sizeof(real_t) == 4 ? 8 : 16)
 #endif
 #ifndef P3DFFT_CT_COL_BYTES
 #define P3DFFT_CT_COL_BYTES 128
 #endif
 #ifndef P3DFFT_CT_NT_MAX
-#define P3DFFT_CT_NT_MAX (sizeof(real_t) == 4 ? 512 : 256)
+#define P3DFFT_CT_NT_MAX (sizeof(real_t) == 4 ? 1024 : 256)
 #endif
 constexpr int kEPTct = P3DFFT_CT_EPT;
 constexpr int pow2floor(int x) { int p = 1; while (p * 2 <= x) p *= 2; return p; }
@@ -1319,15 +1320,19 @@ constexpr int ct_waves(long long lds_bytes, int nt) {
 #ifndef P3DFFT_X_EPT
 #define P3DFFT_X_EPT 16
 #endif
+#ifndef P3DFFT_X_NT_MAX
+#define P3DFFT_X_NT_MAX 512
+#endif
 constexpr int kEPTx = P3DFFT_X_EPT;
 constexpr int ct_x_pairs(int N) {
   int b = 16;
-  while (b > 1 && (b * N > 512 * kEPTx || (long long)(b + 1) * N * (long long)sizeof(cplx_t) > 80 * 1024)) b /= 2;
+  while (b > 1 && (b * N > P3DFFT_X_NT_MAX * kEPTx || (long long)(b + 1) * N * (long long)sizeof(cplx_t) > 80 * 1024))
+    b /= 2;
   return b;
 }
 constexpr int ct_nt_x(int elems) {
   int nt = (elems / kEPTx + 63) / 64 * 64;
-  return nt < 64 ? 64 : (nt > 512 ? 512 : nt);
+  return nt < 64 ? 64 : (nt > P3DFFT_X_NT_MAX ? P3DFFT_X_NT_MAX : nt);
 }
 
 // Twiddles of a specialised length sit in LDS behind the data tile when the
