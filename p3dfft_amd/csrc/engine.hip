@@ -1285,17 +1285,18 @@ static void dispatch_nt(int nt, const A& a, unsigned g, size_t b, hipStream_t s)
 // ---- compile-time specialised lengths ----------------------------------
 // Tuning constants of the specialised kernels (compile-time): complex values
 // per thread, bytes of adjacent columns per row of a column tile, and the
-// largest column workgroup.  The FP32 column passes are bound by the memory
-// requests in flight per CU: over a 512^3 fwd+bwd, 256 threads of 32 values
-// took 1.41 ms, 512 of 16 1.36 ms, 1024 of 8 1.32 ms.
+// largest column workgroup.  The column passes are bound by the memory
+// requests in flight per CU: over a 512^3 fwd+bwd, FP32 256 threads of 32
+// values took 1.41 ms, 512 of 16 1.36 ms, 1024 of 8 1.32 ms; FP64 256 of 16
+// 2.67 ms, 512 of 8 2.54 ms, 1024 of 4 2.54 ms.
 #ifndef P3DFFT_CT_EPT
-#define P3DFFT_CT_EPT (sizeof(real_t) == 4 ? 8 : 16)
+#define P3DFFT_CT_EPT 8
 #endif
 #ifndef P3DFFT_CT_COL_BYTES
 #define P3DFFT_CT_COL_BYTES 128
 #endif
 #ifndef P3DFFT_CT_NT_MAX
-#define P3DFFT_CT_NT_MAX (sizeof(real_t) == 4 ? 1024 : 256)
+#define P3DFFT_CT_NT_MAX (sizeof(real_t) == 4 ? 1024 : 512)
 #endif
 constexpr int kEPTct = P3DFFT_CT_EPT;
 constexpr int pow2floor(int x) { int p = 1; while (p * 2 <= x) p *= 2; return p; }
