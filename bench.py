@@ -197,14 +197,16 @@ def cpu_baseline(n, f, threads, loops=5):
 
 def fp64_poisson(f):
     """FP64 numpy solve of -f_hat/|k|^2 (k=0 -> 0): the check of served results."""
+    import scipy.fft as sf
     nz, ny, nx = f.shape
-    F = np.fft.rfftn(f.astype(np.float64))
+    workers = min(16, len(os.sched_getaffinity(0)))
+    F = sf.rfftn(f.astype(np.float64), workers=workers)
     kx = np.arange(nx // 2 + 1)
     ky = np.fft.fftfreq(ny, 1.0 / ny)
     kz = np.fft.fftfreq(nz, 1.0 / nz)
     k2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
     mult = np.where(k2 > 0, -1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
-    return np.fft.irfftn(F * mult, s=(nz, ny, nx), axes=(0, 1, 2))
+    return sf.irfftn(F * mult, s=(nz, ny, nx), axes=(0, 1, 2), workers=workers)
 
 
 def share_leg(steps=20, warmup=3):
@@ -268,17 +270,20 @@ def serve_leg(n_req, slots, load_frac=0.7):
     try:
         order = list(range(n_req))
         S.fill()
-        # accuracy on a sample, against an FP64 solve of the same right-hand side
-        sample = [i for i in order if np.prod(shapes[i]) <= 128 ** 3][:8]
-        rhs = {i: S.download(i) for i in sample}
-        for i in sample:
-            S.solve_alone(i)
-        err = max(float(np.linalg.norm(S.download(i) - fp64_poisson(rhs[i])) / np.linalg.norm(fp64_poisson(rhs[i])))
-                  for i in sample)
-        S.fill()
         S.run(order[:min(50, n_req)], np.zeros(min(50, n_req)))  # warm
         S.fill()
+        # accuracy of the served results themselves, against FP64 solves of the
+        # same right-hand sides: the 6 largest requests and 6 drawn at random
+        by_size = sorted(order, key=lambda i: -int(np.prod(shapes[i])))
+        rng = np.random.default_rng(7)
+        sample = sorted(set(by_size[:6]) | set(int(i) for i in rng.choice(n_req, 6, replace=False)))
+        rhs = {i: S.download(i) for i in sample}
         _, mk_served, peak = S.run(order, np.zeros(n_req))
+        errs = {i: float(np.linalg.norm(S.download(i) - fp64_poisson(rhs[i])) / np.linalg.norm(fp64_poisson(rhs[i])))
+                for i in sample}
+        err = max(errs.values())
+        err_sample = "served results of %d requests (the 6 largest, up to %s, and 6 at random) against FP64 " \
+                     "solves of the same right-hand sides" % (len(sample), "x".join(map(str, shapes[by_size[0]])))
         S.fill()
         _, mk_seq, _ = S.run(order, np.zeros(n_req), sequential=True)
         thr_served, thr_seq = n_req / mk_served, n_req / mk_seq
@@ -318,8 +323,146 @@ def serve_leg(n_req, slots, load_frac=0.7):
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "one_at_a_time_p50_ms": float(np.percentile(lat_q, 50) * 1e3),
             "one_at_a_time_p99_ms": float(np.percentile(lat_q, 99) * 1e3),
-            "peak_work_bytes": int(peak), "rel_l2_vs_fp64_max": err,
+            "peak_work_bytes": int(peak), "rel_l2_vs_fp64_max": err, "rel_l2_sample": err_sample,
             "mean_points_per_request": float(np.mean([np.prod(s) for s in shapes]))}
+
+
+def c3_leg(steps, warmup, n=384):
+    """The radix-3 precision grid (C3): 384^3 i.i.d. N(0,1), FP32 and FP64
+    fwd+bwd through the device entry points, per-step time, fraction of the
+    compulsory-bytes roofline and the FP32 error against the FP64 path."""
+    f = np.random.default_rng(384).standard_normal((n, n, n))
+    r32 = run_precision(4, n, f, steps, warmup)
+    r64 = run_precision(8, n, f, steps, warmup)
+    nr, nc = n ** 3, (n // 2 + 1) * n * n
+    out = {"grid": [n, n, n], "field": "N(0,1), seed 384"}
+    for name, r, sz in (("fp32", r32, 4), ("fp64", r64, 8)):
+        t = r["ms_per_step"] * 1e-3
+        B2 = 2 * (nr * sz + nc * 2 * sz)
+        out[name] = {"ms_per_step": r["ms_per_step"], "compulsory_bytes": B2, "GBps": B2 / t / 1e9,
+                     "frac_of_8TBps": B2 / t / HBM_PEAK,
+                     "kernels_ms": {SLOT_NAMES[k]: r["timers"][k - 1] / steps * 1e3 for k in SLOT_NAMES}}
+    ref = r64["spec"].astype(np.complex128)
+    out["speedup_fp32_vs_fp64"] = r64["ms_per_step"] / r32["ms_per_step"]
+    out["rel_l2_spectrum_fp32_vs_fp64"] = float(np.linalg.norm(r32["spec"] - ref) / np.linalg.norm(ref))
+    b64 = r64["back"].astype(np.float64)
+    out["rel_l2_roundtrip_fp32_vs_fp64"] = float(np.linalg.norm(r32["back"] - b64) / np.linalg.norm(b64))
+    return out
+
+
+def cheby_field(nx, ny, nz, Lz, seed=257):
+    """Random trig modes in x, y times a random degree-64 Chebyshev polynomial in
+    z (coefficients N(0,1)/(1+n)^2) on the nz Gauss-Lobatto points."""
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal(65) / (1 + np.arange(65)) ** 2
+    zz = 0.5 * Lz * np.cos(np.pi * np.arange(nz) / (nz - 1))
+    g = np.polynomial.chebyshev.chebval(2.0 * zz / Lz, c)
+    xs, ys = np.arange(nx) / nx, np.arange(ny) / ny
+    xy = sum(np.cos(2 * np.pi * (rng.integers(1, 9) * xs[None, :] + rng.integers(1, 9) * ys[:, None])
+                    + rng.random() * 2 * np.pi) for _ in range(4))
+    return g[:, None, None] * xy[None]
+
+
+def cheby_leg(steps=20, warmup=3, cpu_threads=1, cpu=True):
+    """The Chebyshev configuration (C4): 256 x 256 x 257 FP64, p3dfft_cheby
+    ('ffc' forward, scaling, derivative recurrence along z) and the backward
+    btran_c2r 'cff', device entry points, fields resident.  Times of each
+    direction and of the recurrence kernel alone (cheby minus the 'ffc'
+    forward), against the compulsory-bytes roofline; with the CPU oracle's
+    cheby + btran timed on the same field as its baseline and checker."""
+    nx, ny, nz, Lz = 256, 256, 257, 2.0
+    a = cheby_field(nx, ny, nz, Lz)
+    P = p3dfft_amd.P3DFFT(8)
+    P.setup((1, 1), nx, ny, nz)
+    L = P.lib
+    try:
+        fin = P.alloc(P.real_bytes()); spec = P.alloc(P.spec_bytes()); out = P.alloc(P.real_bytes())
+        fin.upload(a)
+        e0, e1 = L.p3dfft_amd_event_create(), L.p3dfft_amd_event_create()
+
+        def timed(fn):
+            for _ in range(warmup):
+                fn()
+            P.synchronize()
+            L.p3dfft_amd_event_record(e0, None)
+            for _ in range(steps):
+                fn()
+            L.p3dfft_amd_event_record(e1, None)
+            return L.p3dfft_amd_event_elapsed_ms(e0, e1) / steps
+        t_cheby = timed(lambda: P.cheby_dev(fin.ptr, spec.ptr, Lz))
+        t_ffc = timed(lambda: P.ftran_r2c_dev(fin.ptr, spec.ptr, "ffc"))
+        P.cheby_dev(fin.ptr, spec.ptr, Lz)
+        got = spec.download(P.spec_shape(), P.cdtype)
+        # btran 'cff' on a copy of the derivative spectrum (overwrite mode
+        # transforms its input in place), restored before each call
+        keep = P.alloc(P.spec_bytes())
+        L.p3dfft_amd_memcpy_d2d(keep.ptr, spec.ptr, P.spec_bytes())
+
+        def bwd():
+            L.p3dfft_amd_memcpy_d2d(spec.ptr, keep.ptr, P.spec_bytes())
+            P.btran_c2r_dev(spec.ptr, out.ptr, "cff")
+        t_b_copy = timed(lambda: L.p3dfft_amd_memcpy_d2d(spec.ptr, keep.ptr, P.spec_bytes()))
+        t_bwd = timed(bwd) - t_b_copy
+        L.p3dfft_amd_memcpy_d2d(spec.ptr, keep.ptr, P.spec_bytes())
+        P.btran_c2r_dev(spec.ptr, out.ptr, "cff")
+        back = out.download(P.real_shape(), P.rdtype)
+        L.p3dfft_amd_event_destroy(e0); L.p3dfft_amd_event_destroy(e1)
+        for b in (fin, spec, out, keep):
+            b.free()
+    finally:
+        P.clean()
+    nr, nc = nx * ny * nz, (nx // 2 + 1) * ny * nz
+    B2 = 2 * (nr * 8 + nc * 16)
+    t = (t_cheby + t_bwd) * 1e-3
+    res = {"grid": [nx, ny, nz], "Lz": Lz, "cheby_ms": t_cheby, "btran_cff_ms": t_bwd,
+           "recurrence_ms": t_cheby - t_ffc, "ms_per_step": t_cheby + t_bwd, "compulsory_bytes": B2,
+           "GBps": B2 / t / 1e9, "frac_of_8TBps": B2 / t / HBM_PEAK,
+           "recurrence_GBps": nc * 32 / ((t_cheby - t_ffc) * 1e-3) / 1e9 if t_cheby > t_ffc else None}
+    if cpu:
+        res["cpu_baseline"] = cpu_baseline_cheby(a, Lz, got, back, cpu_threads)
+    return res
+
+
+def cpu_baseline_cheby(a, Lz, got, back, threads, loops=3):
+    """The FP64 CPU oracle's cheby + btran 'cff' on the same field: its time
+    (median of `loops` after one warm-up) and, as the checker, the GPU
+    results' relative L2 error against its outputs."""
+    from oracle import oracle as O
+    O.set_threads(threads)
+    nz, ny, nx = a.shape
+    p = O.Plan(nx, ny, nz)
+    c = p.cheby(a, Lz)
+    p.btran_c2r(c, "cff")
+    tt = []
+    for _ in range(loops):
+        t0 = time.perf_counter()
+        c = p.cheby(a, Lz)
+        b = p.btran_c2r(c, "cff")
+        tt.append(time.perf_counter() - t0)
+    med = float(np.median(tt))
+    return {"value": 1.0 / med, "unit": "cheby+btran/s", "cores": O.get_threads(), "kind": "port",
+            "sample": "the whole 256x256x257 FP64 case, one warm-up then %d x (cheby, btran 'cff'); median %.3f s"
+                      % (loops, med),
+            "gpu_rel_l2_cheby_vs_oracle": float(np.linalg.norm(got - c) / np.linalg.norm(c)),
+            "gpu_rel_l2_btran_vs_oracle": float(np.linalg.norm(back - b) / np.linalg.norm(b))}
+
+
+def cpu_baseline_small(n, threads, loops=20):
+    """The CPU-runnable 64^3 FP64 case by the CPU oracle, timed like the
+    reference driver (one warm-up pair, then `loops` pairs; the median)."""
+    from oracle import oracle as O
+    O.set_threads(threads)
+    f = np.random.default_rng(64).random((n, n, n))
+    p = O.Plan(n, n, n)
+    p.btran_c2r(p.ftran_r2c(f, "fft"), "tff", keep_input=False)
+    tt = []
+    for _ in range(loops):
+        t0 = time.perf_counter()
+        p.btran_c2r(p.ftran_r2c(f, "fft"), "tff", keep_input=False)
+        tt.append(time.perf_counter() - t0)
+    med = float(np.median(tt))
+    return {"value": 1.0 / med, "unit": "fwd+bwd/s", "cores": O.get_threads(), "kind": "port",
+            "sample": "64^3 U[0,1) FP64, one warm-up pair then %d pairs; median %.4f s" % (loops, med)}
 
 
 def small_leg(n=64, reps=200):
@@ -431,7 +574,10 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
+                    help="threads of the CPU baselines (default: every core this process may run on)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 384^3 leg")
+    ap.add_argument("--no-c4", action="store_true", help="skip the Chebyshev 256x256x257 leg")
     ap.add_argument("--chunk-mb", type=float, default=None,
                     help="X/Y z-chunk size in MiB of intermediate (library default 0 = no chunking)")
     ap.add_argument("--no-timers", action="store_true",
@@ -514,6 +660,16 @@ def main():
         line["rel_l2_roundtrip_fp32_vs_fp64"] = float(np.linalg.norm(r32["back"] - b64) / np.linalg.norm(b64))
     log("64^3 FP64 leg (eager and graph-captured)")
     line["c1_64cubed"] = small_leg()
+    if not args.no_cpu:
+        line["c1_64cubed"]["cpu_baseline"] = cpu_baseline_small(64, args.cpu_threads)
+    if not args.no_c3:
+        log("384^3 leg (FP32 and FP64)")
+        line["c3_384cubed"] = c3_leg(args.steps, args.warmup)
+        log("384^3 fp32 %.4f ms/step" % line["c3_384cubed"]["fp32"]["ms_per_step"])
+    if not args.no_c4:
+        log("Chebyshev 256x256x257 FP64 leg")
+        line["c4_cheby"] = cheby_leg(args.steps, args.warmup, args.cpu_threads, cpu=not args.no_cpu)
+        log("cheby %.4f ms fwd+bwd" % line["c4_cheby"]["ms_per_step"])
     if not args.no_share:
         log("share-mode leg (one rank of 2048^3 on 16x16)")
         line["share"] = share_leg()

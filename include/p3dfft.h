@@ -241,9 +241,6 @@ void p3dfft_amd_clear_error(void);
 /* Bytes of X/Y intermediate per z-plane chunk (default 0 = one chunk):
  * each chunk's intermediate is re-read while it is still in the last-level cache. */
 void p3dfft_amd_set_chunk_bytes(size_t bytes);
-/* Bytes of X/Y intermediate per z-plane chunk (default 0 = one chunk):
- * each chunk's intermediate is re-read while it is still in the last-level cache. */
-void p3dfft_amd_set_chunk_bytes(size_t bytes);
 /* Stage timers on (default) or off. */
 void p3dfft_amd_set_timing(int on);
 /* Compile-time specialised kernels for the lengths that have them (default 1);
@@ -261,12 +258,6 @@ void p3dfft_amd_set_wide_offsets(int on);
  * (the Z pass then reads lines the Y pass left in the last-level cache);
  * 0: X r2c into the work buffer, Y from there into the output. */
 void p3dfft_amd_set_x_into_out(int on);
-/* 1: from the next p3dfft_setup on, the X and Y passes of a direction run as
- * one persistent launch (a work queue per XCD; each z-plane's intermediate is
- * re-read from that XCD's L2) where the grid allows (nx == ny specialised
- * lengths 128..512, unpruned X/Y, standard layout, output not overlapping the
- * input); 0 (default): separate X and Y kernels, which measured 2-3x faster
- * on MI355X at 256^3..512^3. */
 /* Plan mode for the next p3dfft_setup (counterpart of the reference's
  * FFTW_MEASURE planning, build/fft_spec.F90:101-108).  0 (default): the
  * default plan of every length.  1: the setup times candidate plans of each
@@ -279,12 +270,28 @@ void p3dfft_amd_set_plan_mode(int mode);
 int p3dfft_amd_plan_report(char *buf, int len);
 /* default plans again (no active setup and no live Poisson graph) */
 int p3dfft_amd_reset_plans(void);
-void p3dfft_amd_set_fused_xy(int on);
-/* -1: the current setup has no fused passes; else the error word of the last
- * fused launch (0 = none, 1 = a dependency wait timed out).  Synchronises. */
-int p3dfft_amd_fused_xy_status(void);
 /* sizeof(p3dfft_real) of this build: 8 or 4. */
 int p3dfft_amd_precision(void);
+
+/* Processor-grid geometry of the current setup (1x1, or the P1 x P2 grid of
+ * a share-mode setup): the reference's module arrays proc_dims and
+ * proc_id2coords, which build/setup.F90:224-229,551-574 gather from every
+ * rank.  proc_dims[(c-1) + 2*((f-1) + 9*p)] for conf c = 1, 2, field
+ * f = 1..9 (start x,y,z, end x,y,z, size x,y,z as p3dfft_get_dims) and rank
+ * p; id2coords[2p], id2coords[2p+1] = (ipid, jpid) of rank p, with rank
+ * p = jpid * P1 + ipid (the reference's Cartesian order).  Arrays of 18P and
+ * 2P ints (null skips one).  Returns P, or -1 before setup. */
+int p3dfft_amd_proc_dims(int *proc_dims, int *id2coords);
+/* get_proc_parts (build/module.F90:888-1054, a Fortran module procedure
+ * without a C binding in the reference): the pieces of the box with 1-based
+ * base point (base_x, base_y, base_z) and sizes (size_x, size_y, size_z) in
+ * configuration conf (1: real X pencils, 2: complex Z pencils) held by each
+ * rank, in the reference's order.  parts is proc_parts(P, 7) in Fortran
+ * order (parts[(col-1)*P + (q-1)]): column 1 the rank id, 2-4 the base point,
+ * 5-7 the sizes, x-y-z order; unused rows -1.  *ierr = 0, 1 (conf not 1 or
+ * 2) or -1 (box not inside the grid).  Returns the number of pieces or -1. */
+int p3dfft_amd_get_proc_parts(int base_x, int base_y, int base_z, int size_x, int size_y, int size_z, int conf,
+                              int *parts, int *ierr);
 
 /* HBM helpers for callers without their own HIP bindings. */
 void *p3dfft_amd_malloc(size_t bytes);

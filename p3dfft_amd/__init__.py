@@ -88,7 +88,6 @@ def load(precision=8):
     L.p3dfft_amd_set_tile_order.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_wide_offsets.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_x_into_out.argtypes = [ctypes.c_int]
-    L.p3dfft_amd_set_fused_xy.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_plan_mode.argtypes = [ctypes.c_int]
     L.p3dfft_amd_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.POINTER(_vp)]
     L.p3dfft_amd_batch_create.restype = ctypes.c_int
@@ -114,7 +113,10 @@ def load(precision=8):
     L.p3dfft_amd_plan_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.p3dfft_amd_plan_report.restype = ctypes.c_int
     L.p3dfft_amd_reset_plans.restype = ctypes.c_int
-    L.p3dfft_amd_fused_xy_status.restype = ctypes.c_int
+    L.p3dfft_amd_proc_dims.argtypes = [_vp, _vp]
+    L.p3dfft_amd_proc_dims.restype = ctypes.c_int
+    L.p3dfft_amd_get_proc_parts.argtypes = [ctypes.c_int] * 7 + [_vp, _vp]
+    L.p3dfft_amd_get_proc_parts.restype = ctypes.c_int
     L.p3dfft_share_set_fused.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_chunk_bytes.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_malloc.argtypes = [ctypes.c_size_t]
@@ -227,6 +229,32 @@ class P3DFFT:
         c = ctypes.c_int(conf)
         self.lib.p3dfft_get_dims(s, e, z, ctypes.byref(c))
         return tuple(s), tuple(e), tuple(z)
+
+    def proc_dims(self):
+        """(proc_dims as an array (P, 2, 9): rank, conf, field; id2coords (P, 2))
+        of the current setup (the reference's module arrays)."""
+        P = self.lib.p3dfft_amd_proc_dims(None, None)
+        if P < 0:
+            self._raise_if_error(force=True)
+        d = np.zeros(18 * P, dtype=np.int32)
+        c = np.zeros(2 * P, dtype=np.int32)
+        self.lib.p3dfft_amd_proc_dims(d.ctypes.data, c.ctypes.data)
+        return d.reshape(P, 9, 2).transpose(0, 2, 1).copy(), c.reshape(P, 2)
+
+    def get_proc_parts(self, base, size, conf):
+        """get_proc_parts (build/module.F90:888-1054): (ierr, parts) with parts a
+        list of (rank, (base x, y, z), (size x, y, z))."""
+        P = self.lib.p3dfft_amd_proc_dims(None, None)
+        if P < 0:
+            self._raise_if_error(force=True)
+        a = np.zeros(7 * P, dtype=np.int32)
+        ierr = ctypes.c_int(0)
+        n = self.lib.p3dfft_amd_get_proc_parts(*[int(v) for v in base], *[int(v) for v in size], int(conf),
+                                               a.ctypes.data, ctypes.byref(ierr))
+        t = a.reshape(7, P)
+        parts = [(int(t[0, q]), tuple(int(v) for v in t[1:4, q]), tuple(int(v) for v in t[4:7, q]))
+                 for q in range(max(n, 0))]
+        return ierr.value, parts, t
 
     def real_shape(self):
         return (self.nz, self.ny, self.nx)

@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "bin", "amdflang")
 
-SOURCES = ["engine.hip", "share.hip", "api.cpp", "api_share.cpp", "api_poisson.cpp", "serve.cpp"]
+SOURCES = ["engine.hip", "share.hip", "api.cpp", "api_share.cpp", "api_poisson.cpp", "api_procs.cpp", "serve.cpp"]
 # packed-FP32 instructions are disabled for all device code
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -78,6 +78,12 @@ def build_fortran():
         # the Fortran sample driver (examples/driver_sine.f90) against this module
         exe = os.path.join(LIB, "driver_sine_f.x" if not defs else "driver_sine_f_sp.x")
         _run([FLANG, "-cpp", "-O2"] + defs + ["-I", md, os.path.join(HERE, "fortran", "examples", "driver_sine.f90"),
+              "-o", exe, "-L" + LIB, "-l" + os.path.splitext(lib)[0][3:], "-l" + base, "-Wl,-rpath,$ORIGIN"])
+        outs.append(exe)
+        # in-place use through the external wrappers (examples/driver_sine_inplace.f90)
+        exe = os.path.join(LIB, "driver_sine_inplace_f.x" if not defs else "driver_sine_inplace_f_sp.x")
+        _run([FLANG, "-cpp", "-O2"] + defs + ["-I", md,
+              os.path.join(HERE, "fortran", "examples", "driver_sine_inplace.f90"),
               "-o", exe, "-L" + LIB, "-l" + os.path.splitext(lib)[0][3:], "-l" + base, "-Wl,-rpath,$ORIGIN"])
         outs.append(exe)
         # the utility procedures' check (examples/driver_utils.f90)

@@ -23,6 +23,8 @@ namespace p3amd {
 
 State g;
 
+std::atomic<int> g_live_captures{0};
+
 static int g_err_mode = 0;  // 0: print and terminate (reference: MPI_Abort); 1: print and return
 static std::string g_last_error;
 
@@ -172,6 +174,16 @@ void ensure_staging() {
   g.stage_bytes = bytes;
 }
 
+// The Z op's transform length fits the engine: nz for 'f'/'t', the even
+// extension 2(nz-1) for the DCT-I and the odd one 2(nz+1) for the DST-I.
+bool zop_len_ok(int nz, int kind) {
+  const long long m = kind == COL_DCT1 ? 2LL * (nz - 1) : (kind == COL_DST1 ? 2LL * (nz + 1) : (long long)nz);
+  if (m <= max_len()) return true;
+  return fatal(std::string(" p3dfft_amd: the ") + (kind == COL_DCT1 ? "DCT-I" : "DST-I") +
+               " along z of nz = " + std::to_string(nz) + " needs transform length " + std::to_string(m) +
+               ", above the supported " + std::to_string(max_len()));
+}
+
 static int zkind(char c, bool* none) {
   *none = false;
   if (c == 't' || c == 'f') return COL_C2C;
@@ -202,6 +214,7 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
   bool none;
   const int kind = zkind((char)op[2], &none);
   if (kind < 0) return fatal(std::string("Unknown transform type: ") + (char)op[2]);
+  if (!zop_len_ok(S.nz, kind)) return false;
   const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
   const int dny = ny - nyc, dnz = nz - nzc;
   const long long nxy = (long long)nxhpc * nyc;
@@ -224,12 +237,7 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
   // there, so the Y pass reads (and leaves in the last-level cache) the lines
   // the Z pass reads next
   const bool xin = g_x_into_out && direct && dny == 0 && !alias && zc == nz;
-  if (xin && S.fuse && fuse_xy_enabled()) {
-    // X and Y in one persistent launch, each plane's X output re-read from L2
-    tstart(S, 5, s);
-    launch_xy_fwd(S.fuse, in, out, s);
-    tstop(S, 5, s);
-  } else if (xin) {
+  if (xin) {
     tstart(S, 5, s);
     launch_x_r2c(in, out, (long long)ny * nz, nx, nxhpc, nx, nxhpc, s);
     tstop(S, 5, s);
@@ -293,6 +301,7 @@ bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hi
   bool none;
   const int kind = zkind((char)op[0], &none);
   if (kind < 0) return fatal(std::string(" Unknown transform type: ") + (char)op[0]);
+  if (!zop_len_ok(S.nz, kind)) return false;
   const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
   const int dny = ny - nyc, dnz = nz - nzc;
   const long long nxy = (long long)nxhpc * nyc;
@@ -334,20 +343,6 @@ bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hi
     zdst = S.w2;
   }
   tstop(S, 9, s);
-  // unpruned X/Y, output not overlapping the Z result: Y and X c2r in one
-  // persistent launch, each plane's Y output re-read from L2 (ring in w1)
-  if (S.fuse && fuse_xy_enabled() && !S.stride1 && dny == 0 && nxhpc == nx / 2 + 1 &&
-      fuse_ring_elems(S.fuse) <= (long long)nxhpc * ny * nz) {
-    const char* zb = (const char*)zdst;
-    const char* ob = (const char*)out;
-    const bool alias = ob < zb + (size_t)nxy * nz * sizeof(cplx_t) && zb < ob + (size_t)nx * ny * nz * sizeof(real_t);
-    if (!alias) {
-      tstart(S, 12, s);
-      launch_yx_bwd(S.fuse, zdst, out, S.w1, s);
-      tstop(S, 12, s);
-      return true;
-    }
-  }
   // Y c2c backward then X c2r, chunk by chunk over z-planes (the chunk's
   // intermediate stays in the last-level cache); seg_copy_y + seg_zero_y and
   // seg_copy_x + seg_zero_x (x > nxhpc) are folded into the loads
@@ -400,8 +395,14 @@ bool cheby_dev(State& S, const real_t* in, cplx_t* out, double Lz, hipStream_t s
 // j*dim_out (elements of the array's type).  Each stage covers all variables
 // in one launch where the layouts allow it.
 // ---------------------------------------------------------------------------
-void ensure_many(State& S, int nv) {
-  if (nv <= S.nv_alloc) return;
+bool ensure_many(State& S, int nv) {
+  if (nv <= S.nv_alloc) return true;
+  if (g_live_captures > 0) {
+    // a captured graph replays launches on the current work buffers
+    notice(" p3dfft_amd: a _many call with more variables than any earlier one needs larger work buffers; "
+           "destroy the captured graphs first (p3dfft_amd_graph_destroy)");
+    return false;
+  }
   if (S.w1) (void)hipFree(S.w1);
   if (S.w2) (void)hipFree(S.w2);
   S.w1 = S.w2 = nullptr;
@@ -409,6 +410,7 @@ void ensure_many(State& S, int nv) {
   check_hip(hipMalloc(&S.w1, (size_t)S.nxhpc * S.ny * S.nz * nv * sizeof(cplx_t)), "hipMalloc(work)");
   check_hip(hipMalloc(&S.w2, (size_t)S.nxhpc * S.nyc * S.nz * nv * sizeof(cplx_t)), "hipMalloc(work)");
   S.nv_alloc = nv;
+  return true;
 }
 
 bool forward_many_dev(State& S, const real_t* in, long long dim_in, cplx_t* out, long long dim_out, int nv,
@@ -416,8 +418,9 @@ bool forward_many_dev(State& S, const real_t* in, long long dim_in, cplx_t* out,
   bool none;
   const int kind = zkind((char)op[2], &none);
   if (kind < 0) return fatal(std::string("Unknown transform type: ") + (char)op[2]);
+  if (!zop_len_ok(S.nz, kind)) return false;
   if (nv <= 0) return true;
-  ensure_many(S, nv);
+  if (!ensure_many(S, nv)) return false;
   const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
   const long long nxy = (long long)nxhpc * nyc;
   const long long nreal = (long long)nx * ny * nz;
@@ -508,8 +511,9 @@ bool backward_many_dev(State& S, cplx_t* in, long long dim_in, real_t* out, long
   bool none;
   const int kind = zkind((char)op[0], &none);
   if (kind < 0) return fatal(std::string(" Unknown transform type: ") + (char)op[0]);
+  if (!zop_len_ok(S.nz, kind)) return false;
   if (nv <= 0) return true;
-  ensure_many(S, nv);
+  if (!ensure_many(S, nv)) return false;
   const int nx = S.nx, ny = S.ny, nz = S.nz, nxhpc = S.nxhpc, nyc = S.nyc, nzc = S.nzc;
   const int dny = ny - nyc, dnz = nz - nzc;
   const long long nxy = (long long)nxhpc * nyc;
@@ -647,11 +651,22 @@ namespace p3amd {
 // ms per forward+backward pair on scratch arrays (one untimed pair first)
 static double time_pair(State& S, const real_t* rin, cplx_t* cspec, real_t* rout, hipStream_t s) {
   static const unsigned char fop[4] = {'f', 'f', 't', 0}, bop[4] = {'t', 'f', 'f', 0};
-  const bool tm = S.timing;
+  // the stage timers are off while candidates run and the events are
+  // released on every exit, also when a candidate throws
+  struct Guard {
+    State& S;
+    bool tm;
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Guard() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+      S.timing = tm;
+    }
+  } gd{S, S.timing};
   S.timing = false;
-  hipEvent_t a, b;
-  check_hip(hipEventCreate(&a), "hipEventCreate");
-  check_hip(hipEventCreate(&b), "hipEventCreate");
+  check_hip(hipEventCreate(&gd.a), "hipEventCreate");
+  check_hip(hipEventCreate(&gd.b), "hipEventCreate");
+  hipEvent_t a = gd.a, b = gd.b;
   auto run = [&](int reps) {
     check_hip(hipEventRecord(a, s), "hipEventRecord");
     for (int i = 0; i < reps; i++) {
@@ -666,11 +681,7 @@ static double time_pair(State& S, const real_t* rin, cplx_t* cspec, real_t* rout
   };
   const double t1 = run(1);
   const int reps = std::max(2, std::min(50, (int)(5.0 / std::max(t1, 1e-3))));
-  const double t = run(reps);
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  S.timing = tm;
-  return t;
+  return run(reps);
 }
 
 static void measure_plans(State& S) {
@@ -748,8 +759,6 @@ void release_state() {
   g.evpool.clear();
   if (g.w1) (void)hipFree(g.w1);
   if (g.w2) (void)hipFree(g.w2);
-  fuse_plan_free(g.fuse);
-  g.fuse = nullptr;
   if (g.spec_partial) (void)hipFree(g.spec_partial);
   g.spec_partial = nullptr;
   g.spec_partial_bytes = 0;
@@ -809,7 +818,8 @@ void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int
   compute_geometry(g, dims, 0, 0);
   if (g.stride1) printf(" Using stride-1 layout\n");
   const int lim = max_len();
-  if (g.nx > lim || g.ny > lim || 2 * (g.nz + 1) > lim) {
+  // the Z length of a DCT-I / DST-I op is checked when that op is requested
+  if (g.nx > lim || g.ny > lim || g.nz > lim) {
     fatal(" p3dfft_amd: grid dimension exceeds the supported transform length " + std::to_string(lim));
     return;
   }
@@ -821,8 +831,6 @@ void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int
   get_len(g.ny);
   get_len(g.nz);
   if (g_plan_mode == 1) measure_plans(g);
-  if (fuse_xy_enabled() && !g.stride1 && g.nxhpc == g.nx / 2 + 1 && g.nyc == g.ny)
-    g.fuse = fuse_plan_create(g.nx, g.ny, g.nz);
   g.set = true;
   memsize[0] = g.memsize[0];
   memsize[1] = g.memsize[1];
@@ -905,6 +913,19 @@ int p3dfft_power_spectrum(const real_t* B, int kmax, double* E) {
   return -1;
 }
 
+// staging for nv variables of the host-array _many entry points
+static void ensure_staging_many(size_t in_bytes, size_t out_bytes) {
+  const size_t bytes = std::max(in_bytes, out_bytes);
+  if (g.stage_bytes >= bytes) return;
+  if (g.stage_a) (void)hipFree(g.stage_a);
+  if (g.stage_b) (void)hipFree(g.stage_b);
+  g.stage_a = g.stage_b = nullptr;
+  g.stage_bytes = 0;
+  check_hip(hipMalloc(&g.stage_a, bytes), "hipMalloc(staging)");
+  check_hip(hipMalloc(&g.stage_b, bytes), "hipMalloc(staging)");
+  g.stage_bytes = bytes;
+}
+
 void p3dfft_ftran_r2c(real_t* A, real_t* B, unsigned char* op) {
   if (!require_full()) return;
   GUARD_BEGIN
@@ -929,17 +950,13 @@ int p3dfft_ftran_r2c_1d_dev(const real_t* in, real_t* out, void* stream) {
 void p3dfft_ftran_r2c_1d(real_t* A, real_t* B) {
   if (!require_full()) return;
   GUARD_BEGIN
-  ensure_staging();
+  // the staging buffers grow once to the (nxhp, jisize, kjsize) output
   const size_t nout = (size_t)g.nxhp * g.jisize * g.kjsize;
-  struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-  } d_out;
-  check_hip(hipMalloc(&d_out.p, nout * sizeof(cplx_t)), "hipMalloc(ftran_r2c_1d)");
+  ensure_staging_many(real_elems() * sizeof(real_t), nout * sizeof(cplx_t));
   check_hip(hipMemcpy(g.stage_a, A, real_elems() * sizeof(real_t), hipMemcpyHostToDevice), "hipMemcpy H2D");
-  launch_x_r2c((const real_t*)g.stage_a, (cplx_t*)d_out.p, (long long)g.jisize * g.kjsize, g.nx, g.nxhp, g.nx,
+  launch_x_r2c((const real_t*)g.stage_a, (cplx_t*)g.stage_b, (long long)g.jisize * g.kjsize, g.nx, g.nxhp, g.nx,
                g.nxhp, g.stream);
-  check_hip(hipMemcpy(B, d_out.p, nout * sizeof(cplx_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+  check_hip(hipMemcpy(B, g.stage_b, nout * sizeof(cplx_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
   GUARD_END
 }
 
@@ -965,18 +982,6 @@ void p3dfft_cheby(real_t* A, real_t* B, real_t* Lz) {
   GUARD_END
 }
 
-// staging for nv variables of the host-array _many entry points
-static void ensure_staging_many(size_t in_bytes, size_t out_bytes) {
-  const size_t bytes = std::max(in_bytes, out_bytes);
-  if (g.stage_bytes >= bytes) return;
-  if (g.stage_a) (void)hipFree(g.stage_a);
-  if (g.stage_b) (void)hipFree(g.stage_b);
-  g.stage_a = g.stage_b = nullptr;
-  g.stage_bytes = 0;
-  check_hip(hipMalloc(&g.stage_a, bytes), "hipMalloc(staging)");
-  check_hip(hipMalloc(&g.stage_b, bytes), "hipMalloc(staging)");
-  g.stage_bytes = bytes;
-}
 
 // build/ftran.F90:84-100 (BIND(C) p3dfft_ftran_r2c_many) and 104-317
 void p3dfft_ftran_r2c_many(real_t* A, int* dim_in, real_t* B, int* dim_out, int* nv, unsigned char* op) {
@@ -1023,6 +1028,11 @@ void p3dfft_cheby_many(real_t* A, int* dim_in, real_t* B, int* dim_out, int* nv,
 
 // build/module.F90:301-424
 void p3dfft_clean(void) {
+  if (g_live_captures > 0) {
+    // the graphs would replay launches on freed work buffers
+    notice(" p3dfft_clean: destroy the captured graphs first (p3dfft_amd_graph_destroy)");
+    return;
+  }
   GUARD_BEGIN
   release_state();
   GUARD_END
@@ -1219,6 +1229,7 @@ int p3dfft_amd_capture_end(void* stream, void** graph) {
   CapturedGraph* c = new CapturedGraph;
   c->exec = ex;
   g_live_graphs++;
+  g_live_captures++;
   if (graph) *graph = c;
   return 0;
 }
@@ -1242,24 +1253,8 @@ int p3dfft_amd_graph_destroy(void* graph) {
   if (c->exec) (void)hipGraphExecDestroy(c->exec);
   delete c;
   g_live_graphs--;
+  g_live_captures--;
   return 0;
-}
-
-// 1: fused X/Y passes for the next setups, where the shape has them;
-// 0 (default): separate passes.
-void p3dfft_amd_set_fused_xy(int on) { set_fuse_xy(on != 0); }
-
-// -1: the current setup has no fused X/Y passes; otherwise the error word of
-// the last fused launch (0 = none; 1 = a dependency wait timed out).
-// Synchronises the device.
-int p3dfft_amd_fused_xy_status(void) {
-  if (!g.set || !g.fuse) return -1;
-  try {
-    return (int)fuse_status(g.fuse);
-  } catch (const Error& e) {
-    set_error(e.msg);
-    return -2;
-  }
 }
 
 // Device memory helpers for callers without their own HIP runtime bindings.

@@ -108,7 +108,7 @@ int p3dfft_setup_share(int* dims, int* ipid, int* jpid, int* nx, int* ny, int* n
   g.stride1 = false;
   compute_geometry(g, dims, *ipid, *jpid);
   const int lim = max_len();
-  if (g.nx > lim || g.ny > lim || 2 * (g.nz + 1) > lim) {
+  if (g.nx > lim || g.ny > lim || g.nz > lim) {
     fatal(" p3dfft_amd: grid dimension exceeds the supported transform length " + std::to_string(lim));
     return -1;
   }
@@ -265,6 +265,7 @@ int p3dfft_share_ftran_z(p3amd::real_t* recv2, p3amd::real_t* out, const unsigne
   bool none;
   const int kind = zop_kind((char)op[2], &none);
   if (kind < 0) { fatal(std::string("Unknown transform type: ") + (char)op[2]); return -1; }
+  if (!zop_len_ok(g.nz, kind)) return -1;
   hipStream_t st = (hipStream_t)stream;
   const long long nxy = (long long)g.iisize * g.jjsize;
   tstart(g, 8, st);
@@ -293,6 +294,7 @@ int p3dfft_share_btran_z(p3amd::real_t* in, p3amd::real_t* send, const unsigned 
   bool none;
   const int kind = zop_kind((char)op[0], &none);
   if (kind < 0) { fatal(std::string(" Unknown transform type: ") + (char)op[0]); return -1; }
+  if (!zop_len_ok(g.nz, kind)) return -1;
   hipStream_t st = (hipStream_t)stream;
   const long long nxy = (long long)g.iisize * g.jjsize;
   const int dnz = g.nz - g.nzc;

@@ -16,7 +16,7 @@
 //  k_gen_tw          : FP64 twiddle generation (sincospi), stored as double2 or
 //                      as hi+lo float pairs.
 //  k_cheby           : Chebyshev derivative recurrence (build/ftran.F90:408-451)
-//                      in FP64, one thread per (x,y) column.
+//                      in FP64, as two interleaved suffix sums over row chunks.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -715,194 +715,6 @@ __global__ __launch_bounds__(NT, W) void k_zs_grp(ColArgs a, GrpArgs g) {
 }
 
 // ---------------------------------------------------------------------------
-// Fused X and Y passes of one direction: a persistent kernel serving one
-// device work queue per XCD.  Each XCD owns a contiguous block of z-planes;
-// its queue lists, in dependency order, the items of those planes: X items
-// (B row pairs of one plane) and Y items (TX columns of one plane).  A Y item
-// of plane p starts once every X item of p has finished (forward), an X item
-// once every Y item of p has (backward), so a plane's intermediate is written
-// and re-read inside the XCD's own L2 instead of making an HBM round trip.
-// Workgroups take items by ticket; every wait is on an item of a lower ticket
-// of the same queue, already held by a running workgroup, so the waits always
-// end.  They are bounded all the same: a wait past `spin` wall-clock ticks
-// sets the error word and every workgroup stops taking items.
-// ---------------------------------------------------------------------------
-#ifndef P3DFFT_FUSE_RING
-#define P3DFFT_FUSE_RING 3
-#endif
-#ifndef P3DFFT_FUSE_LAG
-#define P3DFFT_FUSE_LAG 1
-#endif
-constexpr int kFuseMaxQ = 16;
-// counters sit 128 bytes apart (one cache line each): counter i at ctr[i * kCtrPad]
-constexpr int kCtrPad = 32;
-struct FuseArgs {
-  unsigned* ctr;      // counters [0,16): queue tickets; 16: error; 17: exits; done X [32,32+nz); done Y after
-  const int* items;   // queue q: items[qoff[q] .. qoff[q+1]), (kind << 30) | (plane << 12) | index
-  int qoff[kFuseMaxQ + 1];
-  int qp0[kFuseMaxQ];  // first plane of each queue
-  int nq, nz, ny, nxh;
-  int nxi, nyi;       // X items / Y items per plane
-  int ring;           // backward: plane slots per queue in `ringbuf`
-  void* ringbuf;
-  long long spin;
-};
-
-__device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Queue tickets and done counts are only touched by the workgroups of one
-// XCD; their atomics run in that XCD's L2.  Polls read at agent scope: a
-// workgroup-scope load may be served by the CU's L1 and never see the count
-// change (measured).
-__device__ __forceinline__ unsigned ld_xcd(const unsigned* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned inc_xcd(unsigned* p) {
-  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// thread 0: wait until *c >= target; false on error or time-out
-__device__ __forceinline__ bool fuse_wait(const unsigned* c, unsigned target, unsigned* err, long long spin) {
-  if (ld_xcd(c) >= target) return true;
-  const unsigned long long t0 = wall_clock64();
-  for (int i = 1;; i++) {
-    __builtin_amdgcn_s_sleep(1);
-    if (ld_xcd(c) >= target) return true;
-    if ((i & 15) == 0 && ld_agent(err)) return false;
-    if ((long long)(wall_clock64() - t0) > spin) {
-      __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-  }
-}
-
-// every thread's stores have reached L2, then thread 0 counts the item done
-__device__ __forceinline__ void fuse_done(unsigned* c) {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) (void)inc_xcd(c);
-}
-
-// TX columns c0.. of plane p of a (nxh, ny, planes) complex array at `in` /
-// `out` whose plane pitches are pin / pout elements
-template <class T, int TX>
-__device__ __forceinline__ ColIO plane_io(const void* in, long long pin, void* out, long long pout, int nxh, int c0) {
-  ColIO io;
-  constexpr long long E = sizeof(cpx<T>);
-  io.in = (const char*)in + (pin + c0) * E;
-  io.out = (char*)out + (pout + c0) * E;
-  io.lane_in = io.lane_out = (unsigned long long)(threadIdx.x % TX) * E;
-  io.sl_in = io.sl_out = (unsigned long long)nxh * E;
-  io.nb = nxh - c0 < TX ? nxh - c0 : TX;
-  io.ld_lo = io.ld_hi = io.nst = io.st_lo = 1 << 30;
-  io.ld_shift = io.st_shift = 0;
-  io.ltab = io.stab = nullptr;
-  io.lane_o = io.lane_i = 0;
-  return io;
-}
-
-// The item bodies are separate (not inlined) functions: inlined into the
-// item loop, their per-thread address arithmetic is hoisted out of the loop
-// and held across it, which spills registers.
-template <class T, bool R2C, int B, int NT, class RL>
-__device__ __noinline__ void fuse_x_item(const void* in, void* out, long long si_row, long long so_row, int nxh,
-                                         FDiv dnxh, const typename TwT<T>::type* tw, long long pin, long long pout,
-                                         int nr) {
-  XArgs a;
-  a.in = in; a.out = out; a.si_row = si_row; a.so_row = so_row; a.nxo = nxh; a.dnxo = dnxh; a.xt = nullptr;
-  if constexpr (R2C) x_r2c_ct_tile<T, B, NT, RL, false>((cpx<T>*)g_smem, tw, a, pin, pout, nr);
-  else x_c2r_ct_tile<T, B, NT, RL, false>((cpx<T>*)g_smem, tw, a, pin, pout, nr);
-}
-
-template <class T, bool FWD, int TX, int NT, class RL>
-__device__ __noinline__ void fuse_y_item(const void* in, long long pin, void* out, long long pout, int nxh, int c0,
-                                         const typename TwT<T>::type* tw) {
-  const ColIO io = plane_io<T, TX>(in, pin, out, pout, nxh, c0);
-  run_fft_io<T, FWD, TX, NT, false, unsigned>((cpx<T>*)g_smem, tw, io, RL{});
-}
-
-// FWD: X r2c of xa.in into xa.out, then Y forward in place there.
-// !FWD: Y backward of xa.in (a spectrum) into the ring, then X c2r into xa.out.
-template <class T, bool FWD, class RL, int NT, int W, bool TWL, int B, int TX>
-__global__ __launch_bounds__(NT, W) void k_xy_ct(XArgs xa, FuseArgs f) {
-  __shared__ int s_item;
-  __shared__ int s_ok;
-  const auto* tw = ct_twiddles<T, NT, TWL>(xa.tw, xa.tw_off, xa.L.twcount);
-  unsigned* err = f.ctr + 16 * kCtrPad;
-  unsigned* done_x = f.ctr + 32 * kCtrPad;
-  unsigned* done_y = done_x + f.nz * kCtrPad;
-  const long long plane = (long long)f.nxh * f.ny;  // complex elements per plane
-  const int q0 = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 15) % f.nq;  // XCC_ID
-  // own queue first; the last workgroup to finish drains any queue no
-  // workgroup served (none when every XCD received workgroups)
-  for (int pass = 0; pass < 2; pass++) {
-    for (int qi = 0; qi < (pass == 0 ? 1 : f.nq); qi++) {
-      const int q = pass == 0 ? q0 : qi;
-      const int qn = f.qoff[q + 1] - f.qoff[q];
-      if (threadIdx.x == 0) {
-        const unsigned t = inc_xcd(f.ctr + q * kCtrPad);
-        s_item = (t < (unsigned)qn && !ld_agent(err)) ? (int)t : -1;
-      }
-      __syncthreads();  // also completes the twiddle staging
-      int t = s_item;
-      while (t >= 0) {
-        unsigned nxt = 0;
-        if (threadIdx.x == 0) nxt = inc_xcd(f.ctr + q * kCtrPad);
-        const int it = f.items[f.qoff[q] + t];
-        const int kind = it >> 30, p = (it >> 12) & 0x3ffff, k = it & 0xfff;
-        const int slot = f.ring > 0 ? q * f.ring + (p - f.qp0[q]) % f.ring : 0;
-        cpx<T>* ring = (cpx<T>*)f.ringbuf + (long long)slot * plane;
-        const bool is_x = kind == 0;
-        // dependency of this item
-        const unsigned* dep = nullptr;
-        unsigned need = 0;
-        if (FWD && !is_x) { dep = done_x + p * kCtrPad; need = (unsigned)f.nxi; }
-        if (!FWD && is_x) { dep = done_y + p * kCtrPad; need = (unsigned)f.nyi; }
-        if (!FWD && !is_x && p - f.ring >= f.qp0[q]) { dep = done_x + (p - f.ring) * kCtrPad; need = (unsigned)f.nxi; }
-        if (threadIdx.x == 0) s_ok = dep ? (int)fuse_wait(dep, need, err, f.spin) : 1;
-        __syncthreads();
-        if (s_ok) {
-          // no stale L1 lines of the plane (an L1 invalidate at workgroup
-          // scope was measured to leave some)
-          if (dep) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          if (is_x) {
-            const long long pr0 = (long long)p * (f.ny / 2) + (long long)k * B;  // first row pair
-            const int nr = (int)((long long)(p + 1) * f.ny - 2 * pr0 < 2 * B ? (long long)(p + 1) * f.ny - 2 * pr0 : 2 * B);
-            if constexpr (FWD)
-              fuse_x_item<T, true, B, NT, RL>(xa.in, xa.out, xa.si_row, xa.so_row, f.nxh, xa.dnxo, tw, pr0, pr0, nr);
-            else
-              fuse_x_item<T, false, B, NT, RL>(ring, xa.out, xa.si_row, xa.so_row, f.nxh, xa.dnxo, tw,
-                                               (long long)k * B, pr0, nr);
-            fuse_done(done_x + p * kCtrPad);
-          } else {
-            if constexpr (FWD) {
-              fuse_y_item<T, true, TX, NT, RL>(xa.out, p * plane, xa.out, p * plane, f.nxh, k * TX, tw);
-            } else {
-              fuse_y_item<T, false, TX, NT, RL>(xa.in, p * plane, ring, 0, f.nxh, k * TX, tw);
-              fuse_done(done_y + p * kCtrPad);
-            }
-          }
-        }
-        __syncthreads();  // LDS free for the next item
-        if (threadIdx.x == 0) s_item = (nxt < (unsigned)qn && !ld_agent(err)) ? (int)nxt : -1;
-        __syncthreads();
-        t = s_item;
-      }
-    }
-    if (pass == 0) {
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        const unsigned e = __hip_atomic_fetch_add(f.ctr + 17 * kCtrPad, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_ok = e + 1 == gridDim.x;
-      }
-      __syncthreads();
-      if (!s_ok) return;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // k2-weighted shell sums of a stored half spectrum (the drivers' power
 // spectrum, sample/C/driver_spec.c:298-383): element (a, b, c), a fastest,
 // adds k2*|B|^2 to shell int(sqrt(k2) + 1/2), k2 = ka^2 + kb^2 + kc^2, where an
@@ -995,16 +807,6 @@ __global__ __launch_bounds__(256) void k_spectrum_sum(const double* partial, int
   if (threadIdx.x == 0) E[blockIdx.x] = red[0];
 }
 
-// XCC_ID of each workgroup (the fused kernels' queue selector)
-__global__ void k_xcc_id(unsigned* out, int n) {
-  if (threadIdx.x == 0 && (int)blockIdx.x < n) out[blockIdx.x] = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15;
-}
-
-__global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 // ---------------------------------------------------------------------------
 // small kernels
 // ---------------------------------------------------------------------------
@@ -1033,34 +835,58 @@ __global__ void k_gen_tw<float4>(const int2* idx, int count, float4* out) {
   out[i] = make_float4(ch, sh, (float)(c - (double)ch), (float)(s - (double)sh));
 }
 
-template <class T>
-__global__ void k_cheby(cpx<T>* o, long long ncol, int N, long long cs, long long ks, double scale,
-                        double L) {
-  const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (c >= ncol || N < 2) return;
-  cpx<T>* p = o + c * cs;
-  // element k (1-based) is p[(k-1)*ks]; the whole recurrence runs in FP64
-  cpx<T> t = p[(long long)(N - 1) * ks];
-  const double aNx = t.x * scale, aNy = t.y * scale;
-  t = p[(long long)(N - 2) * ks];
-  double oldx = t.x * scale, oldy = t.y * scale;
-  const double d1x = L * (double)(N - 1) * aNx * 0.5, d1y = L * (double)(N - 1) * aNy * 0.5;
-  p[(long long)(N - 1) * ks] = mk<T>((T)0, (T)0);
-  if (N == 2) {
-    p[0] = mk<T>((T)(d1x * 0.5), (T)(d1y * 0.5));
-    return;
+// Chebyshev derivative recurrence along z (build/ftran.F90:408-451):
+// with a_{k+1} = scale * row k (0-based rows), d_{N-1} = L (N-1) a_N / 2 and
+// d_k = L k a_{k+1} + d_{k+2}, row r receives d_{r+1} = the sum of
+// L j scale t_j over the rows j > r of the other parity (the top term
+// halved), row 0 is halved again and row N-1 is zero: two interleaved suffix
+// sums.  A workgroup owns TX adjacent columns; P threads per column each
+// own a chunk of rows: pass 1 sums the chunk's terms by parity into LDS, pass
+// 2 walks the chunk from its top with the later chunks' sums as the carry.
+// Sums run in FP64 in both builds; within a chunk they are added in the
+// reference's order (top down).
+template <class T, int TX, int P>
+__global__ __launch_bounds__(TX * P) void k_cheby(cpx<T>* o, long long ncol, int N, long long cs, long long ks,
+                                                 double scale, double L) {
+  __shared__ double2 tot[P][2][TX];
+  const int b = threadIdx.x % TX, q = threadIdx.x / TX;
+  const long long c = (long long)blockIdx.x * TX + b;
+  const bool valid = c < ncol;
+  cpx<T>* p = o + (valid ? c : 0) * cs;
+  const int R = (N + P - 1) / P;
+  const int lo = q * R, hi = min(N, lo + R);
+  auto term = [&](int j, cpx<T> t) {
+    double f = L * (double)j;
+    if (j == N - 1) f *= 0.5;
+    return make_double2(f * ((double)t.x * scale), f * ((double)t.y * scale));
+  };
+  // even-row and odd-row sums (named registers, no indexed private array)
+  double2 ev = make_double2(0, 0), od = make_double2(0, 0);
+  if (valid)
+    for (int j = hi - 1; j >= lo; j--) {
+      const double2 w = term(j, p[(long long)j * ks]);
+      if (j & 1) { od.x += w.x; od.y += w.y; }
+      else { ev.x += w.x; ev.y += w.y; }
+    }
+  tot[q][0][b] = ev;
+  tot[q][1][b] = od;
+  __syncthreads();
+  // carry: the sums of the later chunks, by row parity
+  ev = od = make_double2(0, 0);
+  for (int u = P - 1; u > q; u--) {
+    ev.x += tot[u][0][b].x; ev.y += tot[u][0][b].y;
+    od.x += tot[u][1][b].x; od.y += tot[u][1][b].y;
   }
-  p[(long long)(N - 2) * ks] = mk<T>((T)d1x, (T)d1y);
-  double p2x = 0, p2y = 0, p1x = d1x, p1y = d1y;
-  for (int k = N - 2; k >= 1; k--) {
-    t = p[(long long)(k - 1) * ks];
-    const double akx = t.x * scale, aky = t.y * scale;
-    double dkx = L * (double)k * oldx + p2x, dky = L * (double)k * oldy + p2y;
-    p2x = p1x; p2y = p1y;
-    p1x = dkx; p1y = dky;
-    if (k == 1) { dkx *= 0.5; dky *= 0.5; }
-    p[(long long)(k - 1) * ks] = mk<T>((T)dkx, (T)dky);
-    oldx = akx; oldy = aky;
+  if (!valid) return;
+  for (int r = hi - 1; r >= lo; r--) {
+    cpx<T>* e = p + (long long)r * ks;
+    const cpx<T> t = *e;
+    double2 d = (r & 1) ? ev : od;
+    if (r == 0) { d.x *= 0.5; d.y *= 0.5; }
+    *e = mk<T>((T)d.x, (T)d.y);
+    const double2 w = term(r, t);
+    if (r & 1) { od.x += w.x; od.y += w.y; }
+    else { ev.x += w.x; ev.y += w.y; }
   }
 }
 
@@ -1356,12 +1182,6 @@ struct CtInfo {
   void (*xgrp)(bool, const XArgs&, const GrpArgs&, unsigned, size_t, hipStream_t);
   void (*cgrp)(bool, const ColArgs&, const GrpArgs&, unsigned, size_t, hipStream_t);
   void (*zgrp)(const ColArgs&, const GrpArgs&, unsigned, size_t, hipStream_t);
-  // fused X/Y passes (k_xy_ct): null when not built for this length
-  void (*xy)(bool, const XArgs&, const FuseArgs&, unsigned, size_t, hipStream_t);
-  int BF, NTF;          // row pairs per X item, threads
-  long long DF;         // LDS data bytes
-  bool twl_f;
-  const void* kxy[2];   // the kernels (occupancy queries): backward, forward
 };
 
 template <int... Rs>
@@ -1380,15 +1200,6 @@ struct CtLen {
   // waves per SIMD the LDS footprint allows: the register budget of each kernel
   static constexpr int WC = ct_waves(DC + (TWLC ? TWB : 0), NTC);
   static constexpr int WX = ct_waves(DX + (TWLX ? TWB : 0), NTX);
-  // fused X/Y passes: lengths whose (N/2+1) x N complex plane fits a share of
-  // an XCD's 4 MiB L2 with room for the next plane
-  static constexpr bool FUSE = N >= 128 && N <= 512 && (long long)(N / 2 + 1) * N * (long long)sizeof(cplx_t) <= 1200 * 1024;
-  static constexpr int NTF = NTC;
-  static constexpr int BF = pow2floor(NTF * kEPTx / N) < 1 ? 1 : pow2floor(NTF * kEPTx / N);
-  static constexpr long long DXF = round16((long long)(BF + 1) * N * sizeof(cplx_t));
-  static constexpr long long DF = DXF > DC ? DXF : DC;
-  static constexpr bool TWLF = TWB > 0 && DF + TWB <= kCtLdsTw;
-  static constexpr int WF = ct_waves(DF + (TWLF ? TWB : 0), NTF);
   template <class K>
   static void go(K k, unsigned g, int nt, size_t bytes, hipStream_t s, const ColArgs& a) {
     k<<<g, nt, bytes, s>>>(a);
@@ -1454,17 +1265,7 @@ struct CtLen {
   static void zgrp(const ColArgs& a, const GrpArgs& ga, unsigned g, size_t bytes, hipStream_t s) {
     k_zs_grp<real_t, TX, NTC, RL, WC, TWLC><<<g, NTC, bytes, s>>>(a, ga);
   }
-  static void xy(bool fwd, const XArgs& a, const FuseArgs& f, unsigned g, size_t bytes, hipStream_t s) {
-    if constexpr (FUSE) {
-      if (fwd) k_xy_ct<real_t, true, RL, NTF, WF, TWLF, BF, TX><<<g, NTF, bytes, s>>>(a, f);
-      else k_xy_ct<real_t, false, RL, NTF, WF, TWLF, BF, TX><<<g, NTF, bytes, s>>>(a, f);
-    }
-  }
   static void prep() {
-    if constexpr (FUSE) {
-      allow_big_lds(k_xy_ct<real_t, true, RL, NTF, WF, TWLF, BF, TX>);
-      allow_big_lds(k_xy_ct<real_t, false, RL, NTF, WF, TWLF, BF, TX>);
-    }
     prep_dir<true>();
     prep_dir<false>();
     allow_big_lds(k_x_grp<real_t, true, BX, NTX, RL, WX, TWLX>);
@@ -1488,12 +1289,6 @@ struct CtLen {
     c.twl_col = TWLC; c.twl_x = TWLX; c.twb = TWB;
     c.col = &col; c.colio = &colio; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve; c.prep = &prep;
     c.xgrp = &xgrp; c.cgrp = &cgrp; c.zgrp = &zgrp;
-    if constexpr (FUSE) {
-      c.xy = &xy;
-      c.BF = BF; c.NTF = NTF; c.DF = DF; c.twl_f = TWLF;
-      c.kxy[0] = (const void*)k_xy_ct<real_t, false, RL, NTF, WF, TWLF, BF, TX>;
-      c.kxy[1] = (const void*)k_xy_ct<real_t, true, RL, NTF, WF, TWLF, BF, TX>;
-    }
     return c;
   }
 };
@@ -2035,173 +1830,6 @@ void group_batch_free(GroupBatch* B) {
   delete B;
 }
 
-// ---------------------------------------------------------------------------
-// fused X/Y passes: plan (queues, counters) and launches
-// ---------------------------------------------------------------------------
-struct FusePlan {
-  int n = 0, nz = 0;
-  const CtInfo* ct = nullptr;
-  const Len1D* L = nullptr;
-  int* d_items = nullptr;
-  unsigned* d_ctr = nullptr;
-  int nctr = 0;
-  unsigned grid[2] = {0, 0};  // backward, forward
-  size_t bytes = 0;
-  int tw_lds = 0, tw_off = 0;
-  FuseArgs fa[2];              // backward, forward
-};
-
-static bool g_fuse_xy = false;
-void set_fuse_xy(bool on) { g_fuse_xy = on; }
-bool fuse_xy_enabled() { return g_fuse_xy; }
-
-// number of XCDs: the XCC_ID values a wide launch reports (0 when the
-// register does not give a dense 0..n-1 set)
-static int xcd_count() {
-  static int n = -1;
-  if (n >= 0) return n;
-  const int nwg = 8192;
-  unsigned* d = nullptr;
-  check_hip(hipMalloc(&d, nwg * sizeof(unsigned)), "hipMalloc(xcc)");
-  k_xcc_id<<<nwg, 64>>>(d, nwg);
-  check_hip(hipGetLastError(), "k_xcc_id launch");
-  std::vector<unsigned> h(nwg);
-  check_hip(hipMemcpy(h.data(), d, nwg * sizeof(unsigned), hipMemcpyDeviceToHost), "hipMemcpy(xcc)");
-  (void)hipFree(d);
-  unsigned mx = 0;
-  for (unsigned v : h) mx = std::max(mx, v);
-  std::vector<int> seen(mx + 1, 0);
-  for (unsigned v : h) seen[v] = 1;
-  n = (int)mx + 1;
-  for (int v : seen)
-    if (!v) n = 0;
-  if (n > kFuseMaxQ) n = 0;
-  return n;
-}
-
-// items of one queue (planes p0 .. p1-1) in dependency order: the consumer
-// items of a plane follow its producer items `lag` planes later
-static void fuse_queue(std::vector<int>& it, bool fwd, int p0, int p1, int nxi, int nyi, int lag) {
-  auto put = [&](int kind, int p) {
-    const int cnt = kind == 0 ? nxi : nyi;
-    for (int k = 0; k < cnt; k++) it.push_back((kind << 30) | (p << 12) | k);
-  };
-  const int prod = fwd ? 0 : 1, cons = 1 - prod;
-  for (int p = p0; p < p1; p++) {
-    put(prod, p);
-    if (p - lag >= p0) put(cons, p - lag);
-  }
-  for (int p = std::max(p0, p1 - lag); p < p1; p++) put(cons, p);
-}
-
-constexpr int kFuseRing = P3DFFT_FUSE_RING;
-constexpr int kFuseLag = P3DFFT_FUSE_LAG;
-
-FusePlan* fuse_plan_create(int nx, int ny, int nz) {
-  if (!g_use_ct || nx != ny || (ny & 1) || nz < 1 || nz >= (1 << 18)) return nullptr;
-  const Len1D& L = get_len(nx);
-  const CtInfo* ct = find_ct(L);
-  if (!ct || !ct->xy) return nullptr;
-  const int nq = xcd_count();
-  if (nq < 1) return nullptr;
-  int hdev0 = 0, ncu0 = 0;
-  check_hip(hipGetDevice(&hdev0), "hipGetDevice");
-  check_hip(hipDeviceGetAttribute(&ncu0, hipDeviceAttributeMultiprocessorCount, hdev0), "hipDeviceGetAttribute");
-  // the queues must match the XCDs one to one (32 CUs each on MI355X)
-  if (ncu0 % nq != 0 || ncu0 / nq > 40) return nullptr;
-  const int nxh = nx / 2 + 1;
-  const int nxi = (ny / 2 + ct->BF - 1) / ct->BF;
-  const int nyi = (nxh + ct->TX - 1) / ct->TX;
-  if (nxi >= 4096 || nyi >= 4096) return nullptr;
-  FusePlan* P = new FusePlan;
-  P->n = nx; P->nz = nz; P->ct = ct; P->L = &L;
-  P->bytes = ct_lds((size_t)ct->DF, ct->twl_f, *ct, &P->tw_lds, &P->tw_off);
-  std::vector<int> items;
-  int hdev = 0, ncu = 0, clk_khz = 0;
-  check_hip(hipGetDevice(&hdev), "hipGetDevice");
-  check_hip(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, hdev), "hipDeviceGetAttribute");
-  if (hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, hdev) != hipSuccess || clk_khz <= 0)
-    clk_khz = 100000;
-  for (int d = 0; d < 2; d++) {
-    FuseArgs& f = P->fa[d];
-    std::memset(&f, 0, sizeof(f));
-    f.nq = nq; f.nz = nz; f.ny = ny; f.nxh = nxh; f.nxi = nxi; f.nyi = nyi;
-    f.ring = d == 0 ? kFuseRing : 0;
-    f.spin = (long long)clk_khz * 1000LL * 2;  // two seconds
-    for (int q = 0; q < nq; q++) {
-      const int p0 = (int)((long long)nz * q / nq), p1 = (int)((long long)nz * (q + 1) / nq);
-      f.qp0[q] = p0;
-      f.qoff[q] = (int)items.size();
-      fuse_queue(items, d == 1, p0, p1, nxi, nyi, kFuseLag);
-    }
-    f.qoff[nq] = (int)items.size();
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ct->kxy[d], ct->NTF, P->bytes) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    P->grid[d] = (unsigned)(per_cu * ncu);
-  }
-  check_hip(hipMalloc(&P->d_items, items.size() * sizeof(int)), "hipMalloc(fuse items)");
-  check_hip(hipMemcpy(P->d_items, items.data(), items.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
-  P->nctr = (32 + 2 * nz) * kCtrPad;
-  check_hip(hipMalloc(&P->d_ctr, (size_t)P->nctr * sizeof(unsigned)), "hipMalloc(fuse counters)");
-  for (int d = 0; d < 2; d++) {
-    P->fa[d].items = P->d_items;
-    P->fa[d].ctr = P->d_ctr;
-  }
-  return P;
-}
-
-void fuse_plan_free(FusePlan* P) {
-  if (!P) return;
-  if (P->d_items) (void)hipFree(P->d_items);
-  if (P->d_ctr) (void)hipFree(P->d_ctr);
-  delete P;
-}
-
-long long fuse_ring_elems(const FusePlan* P) {
-  return (long long)P->fa[0].nq * kFuseRing * (P->n / 2 + 1) * (long long)P->n;
-}
-
-// 0: no error recorded by the last fused launch (synchronises the device)
-unsigned fuse_status(const FusePlan* P) {
-  unsigned e = 0;
-  check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
-  check_hip(hipMemcpy(&e, P->d_ctr + 16 * kCtrPad, sizeof(e), hipMemcpyDeviceToHost), "hipMemcpy(fuse status)");
-  return e;
-}
-
-static void fuse_launch(FusePlan* P, bool fwd, const void* in, void* out, void* ring, hipStream_t st) {
-  const int n = P->n, nxh = n / 2 + 1;
-  XArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.in = in; a.out = out;
-  a.nrows = (long long)n * P->nz;
-  a.si_row = fwd ? n : nxh;
-  a.so_row = fwd ? nxh : n;
-  a.nxo = nxh;
-  a.dnxo = make_fdiv(nxh);
-  a.B = P->ct->BF;
-  a.L = P->L->d;
-  a.tw = P->L->tw;
-  a.tw_lds = P->tw_lds;
-  a.tw_off = P->tw_off;
-  FuseArgs f = P->fa[fwd ? 1 : 0];
-  f.ringbuf = ring;
-  k_fill_u32<<<(P->nctr + 255) / 256, 256, 0, st>>>(P->d_ctr, P->nctr, 0u);
-  check_hip(hipGetLastError(), "k_fill_u32 launch");
-  P->ct->xy(fwd, a, f, P->grid[fwd ? 1 : 0], P->bytes, st);
-  check_hip(hipGetLastError(), "k_xy_ct launch");
-}
-
-void launch_xy_fwd(FusePlan* P, const real_t* in, cplx_t* out, hipStream_t st) {
-  fuse_launch(P, true, in, out, nullptr, st);
-}
-
-void launch_yx_bwd(FusePlan* P, const cplx_t* in, real_t* out, cplx_t* ring, hipStream_t st) {
-  fuse_launch(P, false, in, out, ring, st);
-}
-
 // workgroups per plane: at least ~2048 workgroups in all, up to 64 per plane
 static int spectrum_chunks(int d2) { return std::max(1, std::min(64, (2048 + d2 - 1) / std::max(d2, 1))); }
 
@@ -2230,8 +1858,11 @@ void launch_spectrum(const cplx_t* B, int d0, int d1, int d2, int f0, int f1, in
 void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long ks, double scale, double L,
                   hipStream_t st) {
   if (ncol <= 0) return;
-  const int nt = 256;
-  k_cheby<real_t><<<(unsigned)((ncol + nt - 1) / nt), nt, 0, st>>>(out, ncol, nzc, cs, ks, scale, L);
+  // 16 adjacent columns (256 bytes of FP64 complex per row) x 16 row chunks
+  constexpr int TX = 16, P = 16;
+  const long long nwg = (ncol + TX - 1) / TX;
+  if (nwg > 0x7fffffffLL) throw Error{"cheby: grid too large", -1};
+  k_cheby<real_t, TX, P><<<(unsigned)nwg, TX * P, 0, st>>>(out, ncol, nzc, cs, ks, scale, L);
   check_hip(hipGetLastError(), "k_cheby launch");
 }
 

@@ -127,30 +127,12 @@ void group_batch_launch(GroupBatch* B, hipStream_t st);
 int group_batch_launches(const GroupBatch* B);
 void group_batch_free(GroupBatch* B);
 
-// Fused X and Y passes (one persistent launch per direction, a work queue per
-// XCD, each plane's intermediate kept in that XCD's L2; opt-in, off by
-// default).  nullptr when the shape has no fused kernel (nx == ny specialised
-// lengths 128..512 whose plane fits the L2 share).
-struct FusePlan;
-FusePlan* fuse_plan_create(int nx, int ny, int nz);
-void fuse_plan_free(FusePlan* P);
-// complex elements of the backward ring buffer
-long long fuse_ring_elems(const FusePlan* P);
-// error word of the last fused launch (synchronises the device); 0 = none
-unsigned fuse_status(const FusePlan* P);
-// forward: X r2c of in (nx, ny, nz) real, then Y, into out (nx/2+1, ny, nz)
-void launch_xy_fwd(FusePlan* P, const real_t* in, cplx_t* out, hipStream_t st);
-// backward: Y of in (nx/2+1, ny, nz) through `ring`, then X c2r into out;
-// in is left unchanged and must not overlap out
-void launch_yx_bwd(FusePlan* P, const cplx_t* in, real_t* out, cplx_t* ring, hipStream_t st);
 // k2-weighted shell sums of a (d0, d1, d2) half spectrum (a fastest; fold
 // lengths f0, f1, f2, 0 = the axis is not folded) into E[0..kmax], through
 // `partial` (spectrum_partial_bytes)
 size_t spectrum_partial_bytes(int d2, int kmax);
 void launch_spectrum(const cplx_t* B, int d0, int d1, int d2, int f0, int f1, int f2, int kmax, double* partial,
                      double* E, hipStream_t st);
-void set_fuse_xy(bool on);
-bool fuse_xy_enabled();
 
 // Chebyshev post-processing of a 'ffc' spectrum (build/ftran.F90:408-451):
 // scale, then the derivative recurrence along z.  Element (col c, z k) at

@@ -32,8 +32,6 @@ struct State {
   cplx_t* w1 = nullptr;
   cplx_t* w2 = nullptr;
   int nv_alloc = 0;  // variables the work buffers hold (grown by the _many calls)
-  // fused X/Y passes of this grid (null when the shape has none)
-  FusePlan* fuse = nullptr;
   // per-plane partial shell sums of the power spectrum (grown on demand)
   double* spec_partial = nullptr;
   size_t spec_partial_bytes = 0;
@@ -58,8 +56,12 @@ struct State {
 };
 
 extern State g;
-// Poisson graphs alive (plans may not be reset while any exists)
+// Poisson graphs and captured transform graphs alive (plans may not be
+// reset while any exists)
 extern std::atomic<int> g_live_graphs;
+// captured transform graphs alive: they hold the setup's work buffers, so
+// p3dfft_clean and a growth of those buffers are refused while any exists
+extern std::atomic<int> g_live_captures;
 
 bool fatal(const std::string& msg);
 void notice(const std::string& msg);
@@ -77,6 +79,8 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
 bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hipStream_t s);
 bool cheby_dev(State& S, const real_t* in, cplx_t* out, double Lz, hipStream_t s);
 void release_state();
+bool zop_len_ok(int nz, int kind);
+bool ensure_many(State& S, int nv);
 int chunk_planes(State& S);
 void share_release();
 

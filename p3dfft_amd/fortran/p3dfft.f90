@@ -26,10 +26,20 @@ module p3dfft
   ! MPI_DOUBLE_COMPLEX), after which they can be used as in the reference.
   integer, save, public :: p3dfft_mpireal = -1, p3dfft_mpicomplex = -1
 
+  ! processor-grid geometry of the setup (build/module.F90:172-176): every
+  ! rank's get_dims extents proc_dims(conf, field, rank) (fields: start x,y,z,
+  ! end x,y,z, size x,y,z), the rank <-> (ipid, jpid) maps and the result
+  ! array of get_proc_parts
+  integer, save, dimension(:), allocatable, public :: proc_id2coords
+  integer, save, dimension(:, :), allocatable, public :: proc_coords2id
+  integer, save, dimension(:, :, :), allocatable, public :: proc_dims
+  integer, save, dimension(:, :), allocatable, public :: proc_parts
+
   public :: p3dfft_setup, p3dfft_get_dims, p3dfft_ftran_r2c, p3dfft_btran_c2r, &
             p3dfft_cheby, p3dfft_ftran_r2c_many, p3dfft_btran_c2r_many, p3dfft_cheby_many, &
             p3dfft_ftran_r2c_1d, p3dfft_clean, get_timers, set_timers, p3dfft_set_mpi_types, &
-            p3dfft_get_mpi_info, print_buf, print_buf_real, rtran_x2y, rtran_y2x, rtran_x2z, rtran_z2x
+            p3dfft_get_mpi_info, print_buf, print_buf_real, rtran_x2y, rtran_y2x, rtran_x2z, rtran_z2x, &
+            get_proc_parts
 
   ! the communicator handed to p3dfft_setup (one GPU is task 0 of 1)
   integer, save :: setup_comm = -1
@@ -88,6 +98,18 @@ module p3dfft
     end subroutine
     subroutine c_set_timers() bind(C, name='set_timers')
     end subroutine
+    function c_proc_dims(d, c) bind(C, name='p3dfft_amd_proc_dims') result(np)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: d, c
+      integer(c_int) :: np
+    end function
+    function c_get_proc_parts(bx, by, bz, sx, sy, sz, conf, parts, ierr) &
+        bind(C, name='p3dfft_amd_get_proc_parts') result(n)
+      import :: c_ptr, c_int
+      integer(c_int), value :: bx, by, bz, sx, sy, sz, conf
+      type(c_ptr), value :: parts, ierr
+      integer(c_int) :: n
+    end function
   end interface
 
 contains
@@ -113,6 +135,55 @@ contains
     timers = 0.0_r8
     setup_comm = mpi_comm_in
     setup_done = .true.
+    call fill_proc_arrays(dims(1), dims(2))
+  end subroutine
+
+  ! build/setup.F90:224-229, 551-574 (the MPI_Allgather of every rank's
+  ! coordinates and extents), from the library's own record of the splits
+  subroutine fill_proc_arrays(iproc, jproc)
+    integer, intent(in) :: iproc, jproc
+    integer(c_int), target, allocatable :: d(:), c(:)
+    integer :: np, p
+    call free_proc_arrays()
+    np = c_proc_dims(c_null_ptr, c_null_ptr)
+    if (np < 1 .or. np /= iproc * jproc) return
+    allocate (d(18 * np), c(2 * np))
+    np = c_proc_dims(c_loc(d(1)), c_loc(c(1)))
+    allocate (proc_dims(2, 9, 0:np - 1), proc_id2coords(0:2 * np - 1), &
+              proc_coords2id(0:iproc - 1, 0:jproc - 1), proc_parts(np, 7))
+    proc_dims = reshape(d, (/2, 9, np/))
+    proc_id2coords = c
+    do p = 0, np - 1
+      proc_coords2id(c(2 * p + 1), c(2 * p + 2)) = p
+    end do
+    proc_parts = -1
+  end subroutine
+
+  subroutine free_proc_arrays()
+    if (allocated(proc_dims)) deallocate (proc_dims)
+    if (allocated(proc_id2coords)) deallocate (proc_id2coords)
+    if (allocated(proc_coords2id)) deallocate (proc_coords2id)
+    if (allocated(proc_parts)) deallocate (proc_parts)
+  end subroutine
+
+  ! build/module.F90:888-1054: proc_parts(q, :) = (rank, base x, y, z,
+  ! size x, y, z) of the q-th piece of the box held by one rank; unused rows
+  ! -1.  ierr: 0, 1 (conf not 1 or 2), -1 (box not inside the grid).
+  subroutine get_proc_parts(base_x, base_y, base_z, size_x, size_y, size_z, conf, ierr)
+    integer, intent(in) :: base_x, base_y, base_z, size_x, size_y, size_z, conf
+    integer, intent(out) :: ierr
+    integer(c_int), target, allocatable :: a(:)
+    integer(c_int), target :: e
+    integer(c_int) :: n
+    integer :: np
+    ierr = -1
+    if (.not. allocated(proc_parts)) return
+    np = size(proc_parts, 1)
+    allocate (a(7 * np))
+    e = 0
+    n = c_get_proc_parts(base_x, base_y, base_z, size_x, size_y, size_z, conf, c_loc(a(1)), c_loc(e))
+    proc_parts = reshape(a, (/np, 7/))
+    ierr = e
   end subroutine
 
   ! build/module.F90:225-273
@@ -207,6 +278,7 @@ contains
   ! build/module.F90:309
   subroutine p3dfft_clean()
     call c_p3dfft_clean()
+    call free_proc_arrays()
     setup_done = .false.
   end subroutine
 

@@ -22,6 +22,22 @@ def test_fortran_driver_sine(exe):
     assert "Results are correct" in r.stdout
 
 
+@pytest.mark.parametrize("exe", ["driver_sine_inplace_f.x", "driver_sine_inplace_f_sp.x"])
+def test_fortran_driver_sine_inplace(exe):
+    """In-place transforms from Fortran through the external ftran_r2c /
+    btran_c2r (wrap.f90; the reference's build/wrap.F90:82-150), one array
+    passed as input and output as sample/FORTRAN/driver_sine_inplace.F90:206,225
+    does: the four modes of the sine field and the round trip."""
+    path = os.path.join(LIB, exe)
+    if not os.path.exists(path):
+        pytest.fail("Fortran driver not built: %s" % path)
+    r = subprocess.run([path, "32", "24", "20", "2"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0
+    assert "Results are correct" in r.stdout
+
+
 @pytest.mark.parametrize("exe", ["driver_utils_f.x", "driver_utils_f_sp.x"])
 def test_fortran_module_utilities(exe):
     """p3dfft_get_mpi_info, rtran_x2y/y2x/x2z/z2x and print_buf of module

@@ -166,7 +166,7 @@ def test_chunked_xy_passes(precision, stride1):
                     assert rel(back, a) < TOL[precision] * 10
                     buf.free()
     finally:
-        lib.p3dfft_amd_set_chunk_bytes(64 << 20)
+        lib.p3dfft_amd_set_chunk_bytes(0)  # the library default: no chunking
 
 
 # ---------------------------------------------------------------- pruning, layout, OW
@@ -399,6 +399,23 @@ def test_c2_512cube_fp64_spectrum_against_oracle():
     assert rel(s, ref) < 1e-12
 
 
+def test_c3_384cube_against_oracle():
+    """The radix-3 grid 384^3 against the FP64 oracle, so that the length-384
+    column and X kernels are checked against an independent FFT and not only
+    against the GPU's own FP64 path: the FP64 spectrum within 1e-12, the FP32
+    spectrum within 2e-6, and the FP64 round trip of the oracle's spectrum."""
+    n = 384
+    a = np.random.default_rng(385).standard_normal((n, n, n))
+    o = oplan(n, n, n)
+    ref = o.ftran_r2c(a)
+    with lib_setup(8, n, n, n) as P:
+        assert rel(P.ftran_r2c(a), ref) < 1e-12
+        b = P.btran_c2r(ref.copy())
+    assert rel(b, o.btran_c2r(ref)[0]) < 1e-12
+    with lib_setup(4, n, n, n) as P:
+        assert rel(P.ftran_r2c(a.astype(np.float32)), ref) < 2e-6
+
+
 @pytest.mark.parametrize("n", [384, 512])
 def test_c3_precision_fp32_against_fp64(n):
     """FP32 bulk with compensated twiddles vs the FP64 path on i.i.d. N(0,1)."""
@@ -497,62 +514,14 @@ def test_x_into_out_bit_identical(precision, shape, op):
     assert np.array_equal(res[0], res[1])
 
 
-FUSED_SHAPES = [(256, 256, 64), (384, 384, 30), (128, 128, 33), (160, 160, 10), (512, 512, 20), (192, 192, 7)]
-
-
-@pytest.mark.parametrize("precision", [4, 8])
-@pytest.mark.parametrize("shape", FUSED_SHAPES)
-def test_fused_xy_matches_separate_passes_and_oracle(precision, shape):
-    """The fused X/Y launches (opt-in; a work queue per XCD; each plane's X output,
-    forward, or Y output, backward, re-read from L2) give the spectrum and the
-    round trip of the separate X and Y kernels within a tenth of the
-    precision's tolerance, match the oracle, and record no error.  Shapes
-    cover every fused length class, odd plane counts and a plane count below
-    the backward ring (forward fused only)."""
-    nx, ny, nz = shape
-    f = np.random.default_rng(41).standard_normal((nz, ny, nx))
-    res = []
-    for on in (1, 0):
-        P = p3dfft_amd.P3DFFT(precision)
-        P.lib.p3dfft_amd_set_fused_xy(on)
-        P.setup((1, 1), nx, ny, nz)
-        try:
-            s = P.ftran_r2c(f, "fft")
-            b = P.btran_c2r(s, "tff")
-            st = P.lib.p3dfft_amd_fused_xy_status()
-            res.append((s, b, st))
-        finally:
-            P.clean()
-            P.lib.p3dfft_amd_set_fused_xy(0)
-    (s1, b1, st1), (s0, b0, _) = res
-    esize = 8 if precision == 4 else 16
-    if (nx // 2 + 1) * nx * esize > 1200 * 1024:
-        # no fused kernel: the plane does not fit the L2 share; same passes run
-        assert st1 == -1
-        assert np.array_equal(s1, s0) and np.array_equal(b1, b0)
-        return
-    assert st1 == 0, "fused launch reported error word %d" % st1
-    tol = TOL[precision]
-    assert rel(s1, s0) < tol / 10
-    assert rel(b1, b0) < tol / 10
-    if nx * ny * nz <= 256 * 256 * 64:
-        ref = oplan(nx, ny, nz).ftran_r2c(f)
-        assert rel(s1, ref) < tol
-    assert np.abs(b1 / (nx * ny * nz) - f).max() < PREC[precision] * nx * ny * nz * 0.25
-
-
-def test_fused_xy_in_place_backward_uses_separate_passes():
-    """btran with the output over its input (in-place use) keeps the separate
-    passes, whose plane order is safe for the overlap; results match the
-    out-of-place call."""
+def test_in_place_backward_matches_out_of_place():
+    """btran with the output over its input (in-place use of one HBM array)
+    runs the passes in an order that is safe for the overlap; results match
+    the out-of-place call."""
     nx = ny = 256
     nz = 16
     f = np.random.default_rng(43).standard_normal((nz, ny, nx))
-    p3dfft_amd.load(4).p3dfft_amd_set_fused_xy(1)
-    try:
-        _in_place_backward(nx, ny, nz, f)
-    finally:
-        p3dfft_amd.load(4).p3dfft_amd_set_fused_xy(0)
+    _in_place_backward(nx, ny, nz, f)
 
 
 def _in_place_backward(nx, ny, nz, f):
@@ -571,7 +540,6 @@ def _in_place_backward(nx, ny, nz, f):
             L.p3dfft_amd_memcpy_d2h(back.ctypes.data, buf, back.nbytes)
         finally:
             L.p3dfft_amd_free(buf)
-        assert L.p3dfft_amd_fused_xy_status() == 0
     assert rel(back, out_of_place) < TOL[4] / 10
 
 
@@ -768,3 +736,70 @@ def test_batched_transforms_of_mixed_shapes_bit_identical(precision):
                 L.p3dfft_amd_batch_destroy(h)
         for b in reals + specs:
             b.free()
+
+
+def test_clean_and_work_growth_refused_while_a_captured_graph_lives():
+    """A captured transform graph replays launches on the setup's work
+    buffers: while one exists, p3dfft_clean and a _many call that would grow
+    those buffers are refused (with a message) and leave the setup intact;
+    after the graph is destroyed both work."""
+    import ctypes
+    nx, ny, nz = 16, 12, 10
+    f = np.random.default_rng(81).standard_normal((nz, ny, nx))
+    P = p3dfft_amd.P3DFFT(4)
+    P.setup((1, 1), nx, ny, nz)
+    L = P.lib
+    st = L.p3dfft_amd_stream_create()
+    fin, spec, out = P.alloc(P.real_bytes()), P.alloc(P.spec_bytes()), P.alloc(P.real_bytes())
+    graph = ctypes.c_void_p()
+    try:
+        want = P.ftran_r2c(f, "fft")
+        fin.upload(f.astype(P.rdtype))
+        P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft", st)
+        L.p3dfft_amd_stream_synchronize(st)
+        assert L.p3dfft_amd_capture_begin(st) == 0
+        P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft", st)
+        P.btran_c2r_dev(spec.ptr, out.ptr, "tff", st)
+        assert L.p3dfft_amd_capture_end(st, ctypes.byref(graph)) == 0
+        L.p3dfft_amd_clear_error()
+        L.p3dfft_clean()
+        assert "destroy the captured graphs" in P.last_error()
+        L.p3dfft_amd_clear_error()
+        assert rel(P.ftran_r2c(f, "fft"), want) == 0  # the setup is still there
+        n_r, n_c = nx * ny * nz, (nx // 2 + 1) * ny * nz
+        big_in, big_out = P.alloc(2 * n_r * 4), P.alloc(2 * n_c * 8)
+        with pytest.raises(p3dfft_amd.P3DFFTError, match="destroy the captured graphs"):
+            P.ftran_r2c_many_dev(big_in.ptr, n_r, big_out.ptr, n_c, 2, "fft")
+        assert L.p3dfft_amd_graph_launch(graph, st) == 0
+        L.p3dfft_amd_stream_synchronize(st)
+        assert np.array_equal(spec.download(P.spec_shape(), P.cdtype), want.astype(P.cdtype))
+        L.p3dfft_amd_graph_destroy(graph)
+        graph = ctypes.c_void_p()
+        P.ftran_r2c_many_dev(big_in.ptr, n_r, big_out.ptr, n_c, 2, "fft")
+        P.synchronize()
+        big_in.free(); big_out.free()
+    finally:
+        if graph.value:
+            L.p3dfft_amd_graph_destroy(graph)
+        for b in (fin, spec, out):
+            b.free()
+        L.p3dfft_amd_stream_destroy(st)
+        P.clean()
+    assert not P.last_error()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_nz_8192_c2c_thin_grid(precision):
+    """nz up to the engine's largest length (8192) is accepted for the c2c Z
+    ops (setup no longer reserves the DCT-I/DST-I extension for every op):
+    fwd against numpy's rfftn, and the round trip; a DCT-I along that z, which
+    needs length 2(nz-1), is refused with a message."""
+    nx, ny, nz = 4, 3, 8192
+    f = np.random.default_rng(82).standard_normal((nz, ny, nx))
+    with lib_setup(precision, nx, ny, nz) as P:
+        s = P.ftran_r2c(f, "fft")
+        assert rel(s, np.fft.rfftn(f)) < TOL[precision]
+        b = P.btran_c2r(s, "tff") / (nx * ny * nz)
+        assert np.abs(b - f).max() < PREC[precision] * 100
+        with pytest.raises(p3dfft_amd.P3DFFTError, match="needs transform length 16382"):
+            P.ftran_r2c(f, "ffc")

@@ -266,3 +266,186 @@ def test_one_rank_of_2048cube_on_16x16_fp32():
         z.free()
     finally:
         S.clean()
+
+
+def test_one_rank_of_2048cube_on_16x16_fp32_backward():
+    """Rank (15, 3) of 2048^3 on 16x16, backward stages at the real pencil
+    shape: btran_z against numpy's unnormalised inverse FFT along z, btran_y
+    against the oracle's bcomm1 unpack (bcomm1.F90:309-380) + inverse FFT along
+    y + bcomm2 pack (bcomm2.F90:233-273), btran_x against the oracle's bcomm2
+    unpack (bcomm2.F90:290-313) + numpy irfft, on synthetic receive buffers."""
+    n, dims, ip, jp = 2048, (16, 16), 15, 3
+    S = p3dfft_amd.Share(4)
+    S.setup_share(dims, ip, jp, n, n, n)
+    o = O.Plan(n, n, n, dims=dims, ipid=ip, jpid=jp)
+    rng = np.random.default_rng(2049)
+
+    def crandn(m):
+        return (rng.standard_normal(m) + 1j * rng.standard_normal(m)).astype(np.complex64)
+    try:
+        assert (S.iisize, S.jisize, S.jjsize, S.kjsize) == (65, 128, 128, 128)
+        spec = crandn(S.n_zpen())
+        z = dev(S, spec)
+        send = S.alloc(S.n_zpen() * 8)
+        S.btran_z(z, send, "tff")
+        got = get(S, send, S.n_zpen()).reshape(n, 128, 65)
+        ref = np.fft.ifft(spec.astype(np.complex128).reshape(n, 128, 65), axis=0) * n
+        assert rel(got, ref) < TOL[4]
+        z.free(); send.free()
+        del got, ref, spec
+        recv = crandn(S.n_send2())
+        r = dev(S, recv)
+        sendy = S.alloc(S.n_ypen() * 8)
+        S.btran_y(r, sendy)
+        got = get(S, sendy, S.n_ypen())
+        ypen = o.bcomm1_unpack(recv.astype(np.complex128)).reshape(128, n, 65)
+        ref = o.bcomm2_pack(np.fft.ifft(ypen, axis=1) * n)
+        assert ref.size == S.n_ypen()
+        assert rel(got, ref) < TOL[4]
+        r.free(); sendy.free()
+        del got, ref, ypen, recv
+        recv = crandn(S.n_send1())
+        r = dev(S, recv)
+        out = S.alloc(S.n_real() * 4)
+        S.btran_x(r, out)
+        got = out.download((128, 128, n), S.rdtype)
+        xpen = o.bcomm2_unpack(recv.astype(np.complex128)).reshape(128, 128, n // 2 + 1)
+        ref = np.fft.irfft(xpen, n=n, axis=2) * n
+        assert rel(got, ref) < TOL[4]
+        r.free(); out.free()
+    finally:
+        S.clean()
+
+
+def coded_unique(n, seed):
+    """n distinct complex values exact in FP32: the index split into two
+    22-bit halves (a permutation error cannot hide between equal values)."""
+    i = np.arange(n, dtype=np.int64)
+    return (i % (1 << 22) + 1).astype(np.float64) - 1j * ((i >> 22) + 1 + 0.25 * seed)
+
+
+def test_pack_unpack_bit_exact_2048cube_rank_fp32():
+    """The six separate pack/unpack kernels at the real shape of rank (15, 3)
+    of 2048^3 on 16x16 (65-wide last row rank, buffers of 2^24 elements and
+    more) against the oracle's maps, bit-exact on index-coded buffers."""
+    n, dims, ip, jp = 2048, (16, 16), 15, 3
+    S = p3dfft_amd.Share(4)
+    S.setup_share(dims, ip, jp, n, n, n)
+    o = O.Plan(n, n, n, dims=dims, ipid=ip, jpid=jp)
+    try:
+        cases = [
+            (p3dfft_amd.PACK_FCOMM1, S.n_buf2(), S.n_send1(), o.fcomm1_pack),
+            (p3dfft_amd.UNPACK_FCOMM1, S.iisize * n * S.kjsize, S.n_ypen(), o.fcomm1_unpack),
+            (p3dfft_amd.PACK_FCOMM2, S.n_ypen(), S.n_send2(), o.fcomm2_pack),
+            (p3dfft_amd.UNPACK_BCOMM1, S.n_send2(), S.n_ypen(), o.bcomm1_unpack),
+            (p3dfft_amd.PACK_BCOMM2, S.n_ypen(), S.n_ypen(), o.bcomm2_pack),
+            (p3dfft_amd.UNPACK_BCOMM2, S.n_send1(), S.n_buf2(), o.bcomm2_unpack),
+        ]
+        for which, nin, nout, ref_fn in cases:
+            src = coded_unique(nin, which)
+            a = dev(S, src)
+            b = dev(S, np.full(nout, SENT))
+            S.pack(which, a, b)
+            got = get(S, b, nout)
+            ref = ref_fn(src, fill=SENT)
+            assert ref.size == nout
+            assert np.array_equal(got.astype(np.complex128), ref), which
+            a.free(); b.free()
+            del src, got, ref
+    finally:
+        S.clean()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_pack_order_against_hand_worked_buffers(precision):
+    """The GPU fcomm1 and bcomm2 pack kernels reproduce the send buffers of
+    rank (1, 1) of 8 x 6 x 5 on 2 x 2 that were worked out by hand from
+    fcomm1.F90:239-253 and bcomm2.F90:233-273 (tests/golden/hand_pack_2x2.json):
+    the order inside each peer block, independent of the oracle."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "hand_pack_2x2.json")) as f:
+        h = json.load(f)
+    S = p3dfft_amd.Share(precision)
+    S.setup_share(tuple(h["dims"]), h["rank"][0], h["rank"][1], *h["grid"])
+    try:
+        for which, nin, ref in ((p3dfft_amd.PACK_FCOMM1, S.n_buf2(), h["fcomm1_send"]),
+                                (p3dfft_amd.PACK_BCOMM2, S.n_ypen(), h["bcomm2_send"])):
+            a = dev(S, np.arange(nin, dtype=np.float64) + 0j)
+            b = dev(S, np.full(len(ref), SENT))
+            S.pack(which, a, b)
+            got = get(S, b, len(ref))
+            assert np.array_equal(got.real.astype(np.float64), np.array(ref, dtype=np.float64)), which
+            assert np.all(got.imag == 0), which
+            a.free(); b.free()
+    finally:
+        S.clean()
+
+
+def test_proc_dims_and_get_proc_parts():
+    """The processor-grid queries (build/setup.F90:224-229,551-574 and
+    get_proc_parts, build/module.F90:888-1054) on share-mode setups: the
+    proc_dims table seen from one rank equals what every rank's own setup
+    reports through p3dfft_get_dims; rank ids follow the reference's
+    Cartesian order; get_proc_parts matches a hand-worked split and, on
+    random boxes, returns pieces in i-then-j order that each lie inside their
+    rank's extents and tile the box exactly."""
+    S = p3dfft_amd.Share(8)
+    # hand-worked: 8 x 6 x 5 on 2 x 2; y splits (1-3 | 4-6) over ipid, z
+    # splits (1-2 | 3-5) over jpid; rank id = jpid * 2 + ipid
+    S.setup_share((2, 2), 1, 1, 8, 6, 5)
+    try:
+        ierr, parts, _ = S.get_proc_parts((1, 2, 2), (8, 4, 3), 1)
+        assert ierr == 0
+        assert parts == [(0, (1, 2, 2), (8, 2, 1)), (2, (1, 2, 3), (8, 2, 2)),
+                         (1, (1, 4, 2), (8, 2, 1)), (3, (1, 4, 3), (8, 2, 2))]
+        assert S.get_proc_parts((1, 6, 1), (8, 2, 1), 1)[0] == -1
+        assert S.get_proc_parts((1, 1, 1), (1, 1, 1), 3)[0] == 1
+    finally:
+        S.clean()
+    rng = np.random.default_rng(90)
+    for dims, (nx, ny, nz) in (((3, 4), (14, 26, 38)), ((4, 2), (24, 20, 9)), ((1, 3), (8, 9, 10))):
+        own = {}
+        for ip in range(dims[0]):
+            for jp in range(dims[1]):
+                S.setup_share(dims, ip, jp, nx, ny, nz)
+                try:
+                    own[(ip, jp)] = (S.get_dims(1), S.get_dims(2))
+                    pd, coords = S.proc_dims()
+                finally:
+                    S.clean()
+        S.setup_share(dims, dims[0] - 1, 0, nx, ny, nz)
+        try:
+            pd, coords = S.proc_dims()
+            P = dims[0] * dims[1]
+            assert pd.shape == (P, 2, 9)
+            ext = {}
+            for p in range(P):
+                ip, jp = coords[p]
+                assert p == jp * dims[0] + ip
+                for conf in (1, 2):
+                    st, en, sz = own[(ip, jp)][conf - 1]
+                    assert tuple(pd[p, conf - 1]) == st + en + sz, (dims, p, conf)
+                    ext[(p, conf)] = (np.array(st), np.array(en))
+            for conf in (1, 2):
+                full = np.array([nx, ny, nz]) if conf == 1 else np.array([nx // 2 + 1, ny, nz])
+                for _ in range(20):
+                    lo = np.array([rng.integers(1, full[a] + 1) for a in range(3)])
+                    hi = np.array([rng.integers(lo[a], full[a] + 1) for a in range(3)])
+                    ierr, parts, _ = S.get_proc_parts(lo, hi - lo + 1, conf)
+                    assert ierr == 0
+                    cover = np.zeros(tuple(hi - lo + 1), dtype=np.int32)
+                    ia = 1 if conf == 1 else 0
+                    ja = 2 if conf == 1 else 1
+                    prev = None
+                    for rank, b, s in parts:
+                        b, s = np.array(b), np.array(s)
+                        st, en = ext[(rank, conf)]
+                        assert np.all(b >= st) and np.all(b + s - 1 <= en), (conf, rank, b, s)
+                        key = (b[ia], b[ja])
+                        assert prev is None or key > prev  # i outer, then j
+                        prev = key
+                        cover[tuple(slice(b[a] - lo[a], b[a] - lo[a] + s[a]) for a in range(3))] += 1
+                    assert np.all(cover == 1), (dims, conf, lo, hi)
+        finally:
+            S.clean()

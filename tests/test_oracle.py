@@ -430,3 +430,37 @@ def test_power_spectrum_matches_loop_restatement_and_total():
     assert abs(E.sum() - total) < 1e-12 * total
     Es = O.power_spectrum(np.ascontiguousarray(B.transpose(2, 1, 0)), (nx, ny, nz), kmax, stride1=True)
     assert np.allclose(Es, E, rtol=1e-13, atol=0)
+
+
+# Send buffers of rank (ipid, jpid) = (1, 1) of an 8 x 6 x 5 grid on a 2 x 2
+# processor grid, written out by hand from the reference's pack loops
+# (tests/golden/hand_pack_2x2.json; not produced by any code).  nxhp = 5
+# splits over the row as iisz = (2, 3) (remainder to the last rank:
+# iist = (1, 3), iien = (2, 5)); ny = 6 gives jisz = (3, 3) (jist = (1, 4));
+# nz = 5 gives kjsize = 3 for jpid = 1.
+# fcomm1.F90:239-253: for each row peer i, for z = 1..kjsize, y = 1..jisize,
+# x = iist(i)..iien(i): buf1(pos++) = source(x, y, z), peer blocks back to back
+# (IfSndStrt = 0, 18); source (nxhp, jisize, kjsize) = (5, 3, 3).
+# bcomm2.F90:233-273 (non-STRIDE1 branch): for each row peer i, for
+# z = 1..kjsize, position = IfRcvStrt(i) + (z-1) iisize jisz(i), then
+# y = jist(i)..jien(i), x = 1..iisize: buf1(pos++) = source(x, y, z)
+# (IfRcvStrt = 0, 27); source (iisize, ny, kjsize) = (3, 6, 3).
+def hand_pack_fixture():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "hand_pack_2x2.json")) as f:
+        return json.load(f)
+
+
+def test_share_pack_order_against_hand_worked_buffers():
+    """The oracle's fcomm1 and bcomm2 packs reproduce the send buffers worked
+    out by hand from the reference loops, element by element: this pins the
+    order inside each peer block, which the all-ranks round trip cannot see
+    (a pack and an unpack reordered the same way would cancel)."""
+    h = hand_pack_fixture()
+    o = O.Plan(*h["grid"], dims=tuple(h["dims"]), ipid=h["rank"][0], jpid=h["rank"][1])
+    assert (o.iisize, o.jisize, o.kjsize) == (3, 3, 3)
+    src = np.arange(5 * 3 * 3, dtype=np.float64) + 0j
+    assert np.array_equal(o.fcomm1_pack(src).real, np.array(h["fcomm1_send"], dtype=np.float64))
+    ypen = np.arange(3 * 6 * 3, dtype=np.float64) + 0j
+    assert np.array_equal(o.bcomm2_pack(ypen).real, np.array(h["bcomm2_send"], dtype=np.float64))
