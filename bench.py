@@ -437,7 +437,7 @@ def cpu_baseline_cheby(a, Lz, got, back, threads, loops=3):
     for _ in range(loops):
         t0 = time.perf_counter()
         c = p.cheby(a, Lz)
-        b = p.btran_c2r(c, "cff")
+        b = p.btran_c2r(c, "cff")[0]
         tt.append(time.perf_counter() - t0)
     med = float(np.median(tt))
     return {"value": 1.0 / med, "unit": "cheby+btran/s", "cores": O.get_threads(), "kind": "port",

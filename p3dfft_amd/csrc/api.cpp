@@ -210,7 +210,9 @@ static bool g_x_into_out = true;
 // ---------------------------------------------------------------------------
 // forward: X r2c -> Y c2c (per z-plane) -> Z op, build/ftran.F90:489-780
 // ---------------------------------------------------------------------------
-bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s) {
+bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s, double cheby_L,
+                 bool* cheby_done) {
+  if (cheby_done) *cheby_done = false;
   bool none;
   const int kind = zkind((char)op[2], &none);
   if (kind < 0) return fatal(std::string("Unknown transform type: ") + (char)op[2]);
@@ -288,7 +290,13 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
     z.n = nz; z.kind = kind; z.fwd = true;
     z.nld = nz; z.ld_lo = nz; z.ld_hi = nz; z.ld_shift = 0;
     z.nst = nzc; z.st_lo = S.nzcph; z.st_shift = dnz;
-    launch_col(z, s);
+    if (cheby_done && kind == COL_DCT1) {
+      z.cheby = true;
+      z.cheby_scale = 1.0 / ((double)S.nx * (double)S.ny * (double)(S.nzc - 1));
+      z.cheby_L = cheby_L;
+    }
+    const bool done = launch_col(z, s);
+    if (cheby_done) *cheby_done = done && z.cheby;
   }
   tstop(S, 8, s);
   return true;
@@ -381,7 +389,9 @@ bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hi
 bool cheby_dev(State& S, const real_t* in, cplx_t* out, double Lz, hipStream_t s) {
   static const unsigned char ffc[4] = {'f', 'f', 'c', 0};
   if (S.nzc < 2) return fatal("p3dfft_cheby: nzc must be at least 2");
-  if (!forward_dev(S, in, out, ffc, s)) return false;
+  bool fused = false;
+  if (!forward_dev(S, in, out, ffc, s, 4.0 / Lz, &fused)) return false;
+  if (fused) return true;  // the recurrence ran in the DCT-I pass
   const double scale = 1.0 / ((double)S.nx * (double)S.ny * (double)(S.nzc - 1));
   const long long ncol = (long long)S.nxhpc * S.nyc;
   if (S.stride1) launch_cheby(out, ncol, S.nzc, S.nzc, 1, scale, 4.0 / Lz, s);
@@ -771,6 +781,27 @@ void release_state() {
   g.set = false;
 }
 
+// The power spectrum's view of the current setup's spectrum: dims d (first
+// fastest) and fold lengths f (0 = the axis is not folded), after the kmax
+// check of the drivers (driver_spec.F90:272-275).
+struct SpecDims {
+  int d[3], f[3];
+};
+static bool spectrum_dims(int kmax, SpecDims* s) {
+  if (g.stride1) {
+    s->d[0] = g.nzc; s->d[1] = g.nyc; s->d[2] = g.nxhpc; s->f[0] = g.nz; s->f[1] = g.ny; s->f[2] = 0;
+  } else {
+    s->d[0] = g.nxhpc; s->d[1] = g.nyc; s->d[2] = g.nzc; s->f[0] = 0; s->f[1] = g.ny; s->f[2] = g.nz;
+  }
+  double need = 0;
+  for (int i = 0; i < 3; i++) need += (double)(s->d[i] - 1) * (s->d[i] - 1);
+  if (kmax < 0 || 4.0 * kmax * kmax < need) {
+    set_error("Error: kmax is less than it should be (" + std::to_string(kmax) + ")");
+    return false;
+  }
+  return true;
+}
+
 }  // namespace p3amd
 
 // ===========================================================================
@@ -871,17 +902,9 @@ void p3dfft_get_dims(int* start, int* end, int* size, int* conf) {
 int p3dfft_power_spectrum_dev(const real_t* B, int kmax, double* E, void* stream) {
   if (!require_full()) return -1;
   GUARD_BEGIN
-  int d0, d1, d2, f0, f1, f2;
-  if (g.stride1) {
-    d0 = g.nzc; d1 = g.nyc; d2 = g.nxhpc; f0 = g.nz; f1 = g.ny; f2 = 0;
-  } else {
-    d0 = g.nxhpc; d1 = g.nyc; d2 = g.nzc; f0 = 0; f1 = g.ny; f2 = g.nz;
-  }
-  const double need = (double)(d0 - 1) * (d0 - 1) + (double)(d1 - 1) * (d1 - 1) + (double)(d2 - 1) * (d2 - 1);
-  if (kmax < 0 || 4.0 * kmax * kmax < need) {
-    set_error("Error: kmax is less than it should be (" + std::to_string(kmax) + ")");
-    return -1;
-  }
+  SpecDims s;
+  if (!spectrum_dims(kmax, &s)) return -1;
+  const int d0 = s.d[0], d1 = s.d[1], d2 = s.d[2], f0 = s.f[0], f1 = s.f[1], f2 = s.f[2];
   const size_t pb = spectrum_partial_bytes(d2, kmax);
   if (pb > g.spec_partial_bytes) {
     if (g.spec_partial) (void)hipFree(g.spec_partial);

@@ -290,6 +290,11 @@ struct ColArgs {
   const int2* ltab;  // table-addressed rows (see ColIO)
   const int2* stab;
   int tab_off;       // byte offset of the LDS copy of the row tables
+  // Chebyshev recurrence fused into a DCT-I column pass (k_col_ct): applied
+  // to the output rows in LDS before they are stored; cheby_off: byte
+  // offset of the per-chunk sums
+  int cheby, cheby_off;
+  double cheby_scale, cheby_L;
 };
 
 template <class T, bool FWD, int NT, int EPT>
@@ -565,9 +570,57 @@ __global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
     run_fft_io<T, FWD, TX, NT, PR, Off>(lds, tw, io, RL{});
 }
 
+// Chebyshev derivative recurrence (build/ftran.F90:408-451) on the output
+// rows of an LDS column tile: output row e of column b sits in LDS row
+// p(e) = e (e < st_lo) or e + st_shift; the scan is the chunked two-parity
+// suffix sum of k_cheby, with NT / TX chunks per column and the chunk sums
+// in FP64 at `tot` (NT double2).
+template <class T, int TX, int NT>
+__device__ __forceinline__ void cheby_tile(cpx<T>* lds, double2* tot, const ColArgs& a) {
+  constexpr int P = NT / TX;
+  const int N = a.nst;
+  const int b = threadIdx.x % TX, q = threadIdx.x / TX;
+  const int R = (N + P - 1) / P;
+  const int lo = q * R, hi = min(N, lo + R);
+  const double scale = a.cheby_scale, L = a.cheby_L;
+  auto row = [&](int e) { return (e < a.st_lo ? e : e + a.st_shift) * TX + b; };
+  auto term = [&](int j, cpx<T> t) {
+    double f = L * (double)j;
+    if (j == N - 1) f *= 0.5;
+    return make_double2(f * ((double)t.x * scale), f * ((double)t.y * scale));
+  };
+  double2 ev = make_double2(0, 0), od = make_double2(0, 0);
+  for (int j = hi - 1; j >= lo; j--) {
+    const double2 w = term(j, lds[row(j)]);
+    if (j & 1) { od.x += w.x; od.y += w.y; }
+    else { ev.x += w.x; ev.y += w.y; }
+  }
+  tot[(2 * q) * TX + b] = ev;
+  tot[(2 * q + 1) * TX + b] = od;
+  __syncthreads();
+  ev = od = make_double2(0, 0);
+  for (int u = P - 1; u > q; u--) {
+    const double2 e0 = tot[(2 * u) * TX + b], e1 = tot[(2 * u + 1) * TX + b];
+    ev.x += e0.x; ev.y += e0.y;
+    od.x += e1.x; od.y += e1.y;
+  }
+  for (int r = hi - 1; r >= lo; r--) {
+    cpx<T>* e = lds + row(r);
+    const cpx<T> t = *e;
+    double2 d = (r & 1) ? ev : od;
+    if (r == 0) { d.x *= 0.5; d.y *= 0.5; }
+    *e = mk<T>((T)d.x, (T)d.y);
+    const double2 w = term(r, t);
+    if (r & 1) { od.x += w.x; od.y += w.y; }
+    else { ev.x += w.x; ev.y += w.y; }
+  }
+  __syncthreads();
+}
+
 // Column transforms through an LDS load/store pass: DCT-I / DST-I
 // extensions, length-fastest ("STRIDE1") views, copies, and views whose
-// offsets do not fit the 32-bit form.
+// offsets do not fit the 32-bit form.  Loads are issued LU at a time per
+// thread before their LDS stores (several loads in flight per thread).
 template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL>
 __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
   cpx<T>* lds = (cpx<T>*)g_smem;
@@ -580,16 +633,28 @@ __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
   const cpx<T>* in = (const cpx<T>*)a.in + i0 * a.si_in + (long long)o * a.so_in;
   cpx<T>* out = (cpx<T>*)a.out + i0 * a.si_out + (long long)o * a.so_out;
   const int off = (a.kind == COL_DST1) ? 1 : 0;
-  for (int q = threadIdx.x; q < TX * a.nld; q += NT) {
-    int b, e;
-    if (a.ld_lenfast) { b = fdiv(q, a.dnld); e = q - b * a.nld; }
-    else { e = q / TX; b = q % TX; }
-    cpx<T> v = mk<T>(0, 0);
-    if (b < nb) {
+  constexpr int LU = 8;
+  const int nq = TX * a.nld;
+  for (int q0 = threadIdx.x; q0 < nq; q0 += LU * NT) {
+    cpx<T> v[LU];
+    int dst[LU];
+#pragma unroll
+    for (int u = 0; u < LU; u++) {
+      const int q = q0 + u * NT;
+      int b, e;
+      if (a.ld_lenfast) { b = fdiv(q, a.dnld); e = q - b * a.nld; }
+      else { e = q / TX; b = q % TX; }
       const int src = e < a.ld_lo ? e : (e >= a.ld_hi ? e - a.ld_shift : -1);
-      if (src >= 0) v = in[(long long)b * a.si_in + (long long)src * a.sl_in];
+      const bool ok = q < nq && b < nb && src >= 0;
+      // items outside the tile load its first element (always valid) and
+      // are zeroed, so no load sits under a branch
+      const cpx<T> t = in[ok ? (long long)b * a.si_in + (long long)src * a.sl_in : 0];
+      v[u] = ok ? t : mk<T>(0, 0);
+      dst[u] = q < nq ? (e + off) * TX + b : -1;
     }
-    lds[(e + off) * TX + b] = v;
+#pragma unroll
+    for (int u = 0; u < LU; u++)
+      if (dst[u] >= 0) lds[dst[u]] = v[u];
   }
   __syncthreads();
   if (a.kind == COL_DCT1) {
@@ -610,6 +675,7 @@ __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
     __syncthreads();
   }
   if (a.kind != COL_COPY) run_fft_ct<T, FWD, TX, TX, NT>(lds, tw, RL{});
+  if (a.cheby) cheby_tile<T, TX, NT>(lds, (double2*)(g_smem + a.cheby_off), a);
   for (int q = threadIdx.x; q < TX * a.nst; q += NT) {
     int b, e;
     if (a.st_lenfast) { b = fdiv(q, a.dnst); e = q - b * a.nst; }
@@ -1552,8 +1618,8 @@ bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStr
   return true;
 }
 
-void launch_col(const ColSpec& c, hipStream_t st) {
-  if (c.n_inner <= 0 || c.n_outer <= 0 || c.nst <= 0) return;
+bool launch_col(const ColSpec& c, hipStream_t st) {
+  if (c.n_inner <= 0 || c.n_outer <= 0 || c.nst <= 0) return c.cheby;
   ColArgs a;
   std::memset(&a, 0, sizeof(a));
   a.in = c.in; a.out = c.out;
@@ -1604,13 +1670,23 @@ void launch_col(const ColSpec& c, hipStream_t st) {
     if (c.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast && (fits || !pr)) {
       ct->colio(fwd, pr, !fits, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
       check_hip(hipGetLastError(), "k_colio_ct launch");
-      return;
+      return false;
     }
     const long long nwg = (long long)a.tiles * c.n_outer;
     if (nwg > 0x7fffffffLL) throw Error{"grid too large", -1};
-    ct->col(fwd, a, (unsigned)nwg, bytes, st);
+    size_t b2 = bytes;
+    if (c.cheby && c.kind == COL_DCT1) {
+      // per-chunk FP64 sums of the fused recurrence behind the tile
+      a.cheby = 1;
+      a.cheby_off = (int)round16((long long)bytes);
+      a.cheby_scale = c.cheby_scale;
+      a.cheby_L = c.cheby_L;
+      b2 = (size_t)a.cheby_off + (size_t)ct->NTC * 2 * sizeof(double2);
+      if (b2 > kLdsMax) { a.cheby = 0; b2 = bytes; }
+    }
+    ct->col(fwd, a, (unsigned)nwg, b2, st);
     check_hip(hipGetLastError(), "k_col_ct launch");
-    return;
+    return a.cheby != 0;
   }
   const int cpt = L ? len_cap_per_thread(*L) : kEPT;
   // tile width: one segment of adjacent columns (128 bytes unless the plan
@@ -1640,6 +1716,7 @@ void launch_col(const ColSpec& c, hipStream_t st) {
   if (fwd) dispatch_nt<GoCol<true>>(nt, a, (unsigned)nwg, bytes, st);
   else dispatch_nt<GoCol<false>>(nt, a, (unsigned)nwg, bytes, st);
   check_hip(hipGetLastError(), "k_col launch");
+  return false;
 }
 
 bool launch_zsolve(cplx_t* spec, int nxh, int ny, int nz, double scale, hipStream_t st) {

@@ -91,8 +91,14 @@ struct ColSpec {
   int nst;        // rows written to the output
   int st_lo, st_shift;         // output row e <- LDS row e (e<st_lo) or e+st_shift
   bool rev;       // walk the tiles from the last one (see set_tile_order)
+  // DCT-I only: apply the Chebyshev derivative recurrence (scale, L) of
+  // launch_cheby to the stored rows in the same pass where the kernel allows
+  bool cheby = false;
+  double cheby_scale = 0, cheby_L = 0;
 };
-void launch_col(const ColSpec& c, hipStream_t st);
+// Returns true when a requested Chebyshev recurrence was applied (false: the
+// caller runs launch_cheby on the output).
+bool launch_col(const ColSpec& c, hipStream_t st);
 // C2C column transform of n_inner x n_outer columns of length n whose rows
 // are table-addressed: row e of column (o, i) at element ltab[e].x + o *
 // ltab[e].y + i of c.in (zero when ltab[e].x < 0), FFT row p stored at

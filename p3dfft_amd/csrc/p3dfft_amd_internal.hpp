@@ -75,7 +75,10 @@ bool compute_geometry(State& s, const int dims[2], int ipid, int jpid);
 size_t real_elems();
 size_t spec_elems();
 void ensure_staging();
-bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s);
+// cheby_done non-null: fuse the Chebyshev recurrence (L = cheby_L) into the
+// DCT-I Z pass where the kernel allows; *cheby_done tells whether it was
+bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* op, hipStream_t s,
+                 double cheby_L = 0, bool* cheby_done = nullptr);
 bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hipStream_t s);
 bool cheby_dev(State& S, const real_t* in, cplx_t* out, double Lz, hipStream_t s);
 void release_state();

@@ -804,3 +804,32 @@ def test_nz_8192_c2c_thin_grid(precision):
         with pytest.raises(p3dfft_amd.P3DFFTError, match="needs transform length 16382"):
             P.ftran_r2c(f, "ffc")
 
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape,nzc,stride1", [((16, 12, 33), None, False), ((24, 20, 129), None, True),
+                                               ((20, 16, 65), 40, False), ((8, 6, 257), None, False)])
+def test_cheby_recurrence_in_the_dct_pass(precision, shape, nzc, stride1):
+    """p3dfft_cheby with the derivative recurrence applied inside the DCT-I
+    Z pass (specialised kernels, DCT lengths 32 ... 512) against the oracle
+    and against the separate recurrence kernel (run-time-generic kernels),
+    also in the STRIDE1 layout and with a Z cut."""
+    nx, ny, nz = shape
+    Lz = 2.0
+    rng = np.random.default_rng(nz + nx)
+    a = rng.standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz, nzc=nzc, stride1=stride1)
+    ref = o.cheby(a, Lz)
+    lib = p3dfft_amd.load(precision)
+    got = []
+    try:
+        for spec_on in (1, 0):
+            lib.p3dfft_amd_set_specialised(spec_on)
+            with lib_setup(precision, nx, ny, nz, nzc=nzc, stride1=stride1) as P:
+                got.append(P.cheby(a, Lz))
+    finally:
+        lib.p3dfft_amd_set_specialised(1)
+    tol = cheby_tol(precision, nzc or nz, Lz)
+    assert rel(got[0], ref) < tol
+    assert rel(got[1], ref) < tol
+    assert rel(got[0], got[1]) < tol
