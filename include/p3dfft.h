@@ -128,6 +128,15 @@ int p3dfft_cheby_many_dev(const p3dfft_real *in, long long dim_in, p3dfft_real *
  * or -1 when 4*kmax^2 is below the sum of (size-1)^2 over the three axes. */
 int p3dfft_power_spectrum_dev(const p3dfft_real *B, int kmax, double *E, void *stream);
 int p3dfft_power_spectrum(const p3dfft_real *B, int kmax, double *E);
+/* Allocation-free p3dfft_power_spectrum_dev.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-workgroup partial sums go to a static 16 MiB device array that each
+ * GPU holds until the process ends; a setup and kmax whose partials need
+ * more is refused (-1, message) and p3dfft_power_spectrum_dev serves it.
+ * Results are identical to p3dfft_power_spectrum_dev.  Calls that overlap on
+ * one GPU (e.g. on two streams) share the array and race. */
+int p3dfft_power_spectrum_dev_noalloc(const p3dfft_real *B, int kmax, double *E, void *stream);
 
 /* ---- share mode: one rank of a P1 x P2 grid (no counterpart entry points in
  * the reference; they split build/ftran.F90:489-780 and build/btran.F90:396-679

@@ -110,6 +110,8 @@ def load(precision=8):
     L.p3dfft_power_spectrum.restype = ctypes.c_int
     L.p3dfft_power_spectrum_dev.argtypes = [_vp, ctypes.c_int, _vp, _vp]
     L.p3dfft_power_spectrum_dev.restype = ctypes.c_int
+    L.p3dfft_power_spectrum_dev_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp]
+    L.p3dfft_power_spectrum_dev_noalloc.restype = ctypes.c_int
     L.p3dfft_amd_plan_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.p3dfft_amd_plan_report.restype = ctypes.c_int
     L.p3dfft_amd_reset_plans.restype = ctypes.c_int

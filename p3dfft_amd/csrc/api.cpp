@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -802,6 +803,39 @@ static bool spectrum_dims(int kmax, SpecDims* s) {
   return true;
 }
 
+// Static device scratch of the allocation-free entry points: each GPU the
+// library runs on holds its own copy of this array until the process ends.
+// Host code must not use the array's name as a device address: its address
+// on each device comes from hipGetSymbolAddress, taken once per device with
+// that device current (at p3dfft_setup, or at the first call on a device)
+// and kept in g_scratch_dev.
+constexpr size_t kScratchBytes = 16u << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+// the scratch array's address on the current device (null when it cannot be
+// had, with the error recorded if `record`)
+double* scratch_ptr(bool record = true) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    if (record) set_error("p3dfft_amd: no usable current device for the scratch array");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      if (record) set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[dev] = (double*)p;
+  }
+  return g_scratch_dev[dev];
+}
+
 }  // namespace p3amd
 
 // ===========================================================================
@@ -857,6 +891,9 @@ void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int
   check_hip(hipMalloc(&g.w1, (size_t)g.nxhpc * g.ny * g.nz * sizeof(cplx_t)), "hipMalloc(work)");
   check_hip(hipMalloc(&g.w2, (size_t)g.nxhpc * g.nyc * g.nz * sizeof(cplx_t)), "hipMalloc(work)");
   g.nv_alloc = 1;
+  // the scratch array's address on this device, for the allocation-free
+  // entry points (a failure is reported by those entry points)
+  (void)scratch_ptr(false);
   // build the 1D plans (twiddles are generated on the GPU here, once)
   get_len(g.nx);
   get_len(g.ny);
@@ -914,6 +951,34 @@ int p3dfft_power_spectrum_dev(const real_t* B, int kmax, double* E, void* stream
     g.spec_partial_bytes = pb;
   }
   launch_spectrum((const cplx_t*)B, d0, d1, d2, f0, f1, f2, kmax, g.spec_partial, E, (hipStream_t)stream);
+  return 0;
+  GUARD_END
+  return -1;
+}
+
+// Allocation-free variant of p3dfft_power_spectrum_dev (no counterpart in the
+// reference; for callers that cannot allocate device memory: code under
+// stream capture, serving loops, harnesses that forbid it).  The partial
+// shell sums go to the library's static per-GPU scratch array instead of an
+// allocation; a setup whose partials need more than its 16 MiB is refused.
+// The sums do not depend on the array's previous contents.  Calls that
+// overlap on one GPU share the array and race.
+int p3dfft_power_spectrum_dev_noalloc(const real_t* B, int kmax, double* E, void* stream) {
+  if (!require_full()) return -1;
+  GUARD_BEGIN
+  SpecDims s;
+  if (!spectrum_dims(kmax, &s)) return -1;
+  const size_t pb = spectrum_partial_bytes(s.d[2], kmax);
+  if (pb > kScratchBytes) {
+    set_error("p3dfft_power_spectrum_dev_noalloc: the partial sums need " + std::to_string(pb) +
+              " bytes, above the " + std::to_string(kScratchBytes) + "-byte scratch array; use "
+              "p3dfft_power_spectrum_dev");
+    return -1;
+  }
+  double* partial = scratch_ptr();
+  if (!partial) return -1;
+  launch_spectrum((const cplx_t*)B, s.d[0], s.d[1], s.d[2], s.f[0], s.f[1], s.f[2], kmax, partial, E,
+                  (hipStream_t)stream);
   return 0;
   GUARD_END
   return -1;
