@@ -291,8 +291,8 @@ struct ColArgs {
   const int2* stab;
   int tab_off;       // byte offset of the LDS copy of the row tables
   // Chebyshev recurrence fused into a DCT-I column pass (k_col_ct): applied
-  // to the output rows in LDS before they are stored; cheby_off: byte
-  // offset of the per-chunk sums
+  // to the output rows in LDS before they are stored; cheby_off: byte offset
+  // of its per-wave sums behind the tile
   int cheby, cheby_off;
   double cheby_scale, cheby_L;
 };
@@ -572,47 +572,81 @@ __global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
 
 // Chebyshev derivative recurrence (build/ftran.F90:408-451) on the output
 // rows of an LDS column tile: output row e of column b sits in LDS row
-// p(e) = e (e < st_lo) or e + st_shift; the scan is the chunked two-parity
-// suffix sum of k_cheby, with NT / TX chunks per column and the chunk sums
-// in FP64 at `tot` (NT double2).
+// p(e) = e (e < st_lo) or e + st_shift.  As in k_cheby, two interleaved
+// suffix sums: thread (chunk q, column b), b fastest, owns a chunk of at
+// most RM rows of its column, held in registers; chunk sums by parity, then
+// the sums of the later chunks (inside a wave by shuffles at a stride of TX
+// lanes, across waves through LDS at `wt`), then each chunk walked from its
+// top with that carry.  Columns fastest keep the lanes of one LDS access on
+// different banks; every LDS access of a thread is issued as a batch.
+constexpr int kChebyRM = 8;
 template <class T, int TX, int NT>
-__device__ __forceinline__ void cheby_tile(cpx<T>* lds, double2* tot, const ColArgs& a) {
-  constexpr int P = NT / TX;
+__device__ __forceinline__ void cheby_tile(cpx<T>* lds, double* wt, const ColArgs& a) {
+  static_assert(64 % TX == 0 && NT % 64 == 0, "whole column groups per wave");
+  constexpr int P = NT / TX, CW = 64 / TX, NW = NT / 64, RM = kChebyRM;
   const int N = a.nst;
   const int b = threadIdx.x % TX, q = threadIdx.x / TX;
-  const int R = (N + P - 1) / P;
+  const int c = q % CW, w = q / CW;  // chunk within the wave, wave
+  const int R = (N + P - 1) / P;     // <= RM (checked by the host)
   const int lo = q * R, hi = min(N, lo + R);
   const double scale = a.cheby_scale, L = a.cheby_L;
-  auto row = [&](int e) { return (e < a.st_lo ? e : e + a.st_shift) * TX + b; };
-  auto term = [&](int j, cpx<T> t) {
-    double f = L * (double)j;
+  cpx<T> t[RM];
+  int addr[RM];
+#pragma unroll
+  for (int i = 0; i < RM; i++) {
+    const int e = lo + i;
+    addr[i] = (e < a.st_lo ? e : e + a.st_shift) * TX + b;
+    t[i] = (e < hi) ? lds[addr[i]] : mk<T>(0, 0);
+  }
+  // terms L j scale t_j (the top one halved), summed by parity
+  double2 x[RM];
+  double v[4] = {0, 0, 0, 0};  // even re, even im, odd re, odd im
+#pragma unroll
+  for (int i = RM - 1; i >= 0; i--) {
+    const int j = lo + i;
+    double f = (j < hi) ? L * (double)j : 0.0;
     if (j == N - 1) f *= 0.5;
-    return make_double2(f * ((double)t.x * scale), f * ((double)t.y * scale));
-  };
-  double2 ev = make_double2(0, 0), od = make_double2(0, 0);
-  for (int j = hi - 1; j >= lo; j--) {
-    const double2 w = term(j, lds[row(j)]);
-    if (j & 1) { od.x += w.x; od.y += w.y; }
-    else { ev.x += w.x; ev.y += w.y; }
+    x[i] = make_double2(f * ((double)t[i].x * scale), f * ((double)t[i].y * scale));
+    if (j & 1) { v[2] += x[i].x; v[3] += x[i].y; }
+    else { v[0] += x[i].x; v[1] += x[i].y; }
   }
-  tot[(2 * q) * TX + b] = ev;
-  tot[(2 * q + 1) * TX + b] = od;
+  // inclusive suffix over the chunks of this wave (lanes TX apart)
+  double s[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    s[k] = v[k];
+#pragma unroll
+    for (int d = 1; d < CW; d *= 2) {
+      const double u = __shfl_down(s[k], d * TX, 64);
+      if (c + d < CW) s[k] += u;
+    }
+  }
+  if (c == 0)
+#pragma unroll
+    for (int k = 0; k < 4; k++) wt[(w * TX + b) * 4 + k] = s[k];
   __syncthreads();
-  ev = od = make_double2(0, 0);
-  for (int u = P - 1; u > q; u--) {
-    const double2 e0 = tot[(2 * u) * TX + b], e1 = tot[(2 * u + 1) * TX + b];
-    ev.x += e0.x; ev.y += e0.y;
-    od.x += e1.x; od.y += e1.y;
+  double cr[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const double nx = __shfl_down(s[k], TX, 64);
+    cr[k] = c + 1 < CW ? nx : 0.0;
   }
-  for (int r = hi - 1; r >= lo; r--) {
-    cpx<T>* e = lds + row(r);
-    const cpx<T> t = *e;
-    double2 d = (r & 1) ? ev : od;
-    if (r == 0) { d.x *= 0.5; d.y *= 0.5; }
-    *e = mk<T>((T)d.x, (T)d.y);
-    const double2 w = term(r, t);
-    if (r & 1) { od.x += w.x; od.y += w.y; }
-    else { ev.x += w.x; ev.y += w.y; }
+#pragma unroll
+  for (int u = 1; u < NW; u++)
+    if (u > w)
+#pragma unroll
+      for (int k = 0; k < 4; k++) cr[k] += wt[(u * TX + b) * 4 + k];
+  double2 ev = make_double2(cr[0], cr[1]), od = make_double2(cr[2], cr[3]);
+#pragma unroll
+  for (int i = RM - 1; i >= 0; i--) {
+    const int r = lo + i;
+    if (r < hi) {
+      double2 d = (r & 1) ? ev : od;
+      if (r == 0) { d.x *= 0.5; d.y *= 0.5; }
+      lds[addr[i]] = mk<T>((T)d.x, (T)d.y);
+      if (r & 1) { od.x += x[i].x; od.y += x[i].y; }
+      else { ev.x += x[i].x; ev.y += x[i].y; }
+    }
   }
   __syncthreads();
 }
@@ -675,7 +709,9 @@ __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
     __syncthreads();
   }
   if (a.kind != COL_COPY) run_fft_ct<T, FWD, TX, TX, NT>(lds, tw, RL{});
-  if (a.cheby) cheby_tile<T, TX, NT>(lds, (double2*)(g_smem + a.cheby_off), a);
+  if constexpr (64 % TX == 0 && NT % 64 == 0) {
+    if (a.cheby) cheby_tile<T, TX, NT>(lds, (double*)(g_smem + a.cheby_off), a);
+  }
   for (int q = threadIdx.x; q < TX * a.nst; q += NT) {
     int b, e;
     if (a.st_lenfast) { b = fdiv(q, a.dnst); e = q - b * a.nst; }
@@ -1206,17 +1242,24 @@ static void dispatch_nt(int nt, const A& a, unsigned g, size_t b, hipStream_t s)
 #define P3DFFT_CT_NT_MAX (sizeof(real_t) == 4 ? 1024 : 512)
 #endif
 constexpr int kEPTct = P3DFFT_CT_EPT;
+#ifndef P3DFFT_CT_LONG
+#define P3DFFT_CT_LONG 2048
+#endif
 constexpr int pow2floor(int x) { int p = 1; while (p * 2 <= x) p *= 2; return p; }
+// values per thread of the column kernels of length N: lengths of 2048 and
+// more take twice the default, so that a tile still spans 8 columns
+constexpr int ct_ept_col(int N) { return N >= P3DFFT_CT_LONG ? 2 * kEPTct : kEPTct; }
 constexpr int ct_col_tx(int N) {
   int tx = P3DFFT_CT_COL_BYTES / (int)sizeof(cplx_t);
-  int cap = pow2floor(P3DFFT_CT_NT_MAX * kEPTct / N);
+  int cap = pow2floor(P3DFFT_CT_NT_MAX * ct_ept_col(N) / N);
   if (cap < 1) cap = 1;
   return tx < cap ? tx : cap;
 }
-constexpr int ct_nt(int elems) {
-  int nt = (elems / kEPTct + 63) / 64 * 64;
+constexpr int ct_nt_ept(int elems, int ept) {
+  int nt = (elems / ept + 63) / 64 * 64;
   return nt < 64 ? 64 : (nt > P3DFFT_CT_NT_MAX ? P3DFFT_CT_NT_MAX : nt);
 }
+constexpr int ct_nt(int elems) { return ct_nt_ept(elems, kEPTct); }
 constexpr int ct_waves(long long lds_bytes, int nt) {
   long long wg = lds_bytes > 0 ? (160 * 1024) / lds_bytes : 8;
   if (wg < 1) wg = 1;
@@ -1238,8 +1281,12 @@ constexpr int ct_x_pairs(int N) {
     b /= 2;
   return b;
 }
+// a tile that gives fewer than P3DFFT_X_NT_MAX threads at kEPTx values each
+// takes up to twice the threads at half the values: more waves per CU where
+// the LDS allows few workgroups
 constexpr int ct_nt_x(int elems) {
   int nt = (elems / kEPTx + 63) / 64 * 64;
+  if (nt < P3DFFT_X_NT_MAX) nt = (elems / (kEPTx / 2) + 63) / 64 * 64;
   return nt < 64 ? 64 : (nt > P3DFFT_X_NT_MAX ? P3DFFT_X_NT_MAX : nt);
 }
 
@@ -1270,7 +1317,7 @@ struct CtLen {
   typedef RList<Rs...> RL;
   static constexpr int N = RProd<Rs...>::value;
   static constexpr int TX = ct_col_tx(N);
-  static constexpr int NTC = ct_nt(TX * N);
+  static constexpr int NTC = ct_nt_ept(TX * N, ct_ept_col(N));
   static constexpr int BX = ct_x_pairs(N);
   static constexpr int NTX = ct_nt_x(BX * N);
   static constexpr long long TWB = (long long)TwCount<Rs...>::value() * sizeof(Tw);
@@ -1675,13 +1722,14 @@ bool launch_col(const ColSpec& c, hipStream_t st) {
     const long long nwg = (long long)a.tiles * c.n_outer;
     if (nwg > 0x7fffffffLL) throw Error{"grid too large", -1};
     size_t b2 = bytes;
-    if (c.cheby && c.kind == COL_DCT1) {
-      // per-chunk FP64 sums of the fused recurrence behind the tile
+    if (c.cheby && c.kind == COL_DCT1 && 64 % TX == 0 && ct->NTC % 64 == 0 &&
+        (c.nst + ct->NTC / TX - 1) / (ct->NTC / TX) <= kChebyRM) {
+      // per-wave FP64 sums of the fused recurrence behind the tile
       a.cheby = 1;
       a.cheby_off = (int)round16((long long)bytes);
       a.cheby_scale = c.cheby_scale;
       a.cheby_L = c.cheby_L;
-      b2 = (size_t)a.cheby_off + (size_t)ct->NTC * 2 * sizeof(double2);
+      b2 = (size_t)a.cheby_off + (size_t)(ct->NTC / 64) * TX * 4 * sizeof(double);
       if (b2 > kLdsMax) { a.cheby = 0; b2 = bytes; }
     }
     ct->col(fwd, a, (unsigned)nwg, b2, st);
