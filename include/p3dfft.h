@@ -262,6 +262,11 @@ void p3dfft_amd_set_tile_order(int alternate);
 /* 1: the specialised column kernels use 64-bit element offsets even where
  * 32-bit ones fit (default 0; views over 4 GiB always use 64-bit offsets). */
 void p3dfft_amd_set_wide_offsets(int on);
+/* Tile order of the column passes over one contiguous inner axis (the Z
+ * pass): 0 XCD-aware (each XCD a contiguous range of tiles), 1 dispatch
+ * order (the tiles running at once cover one contiguous range of each row),
+ * 2 (default) chosen by the view's shape.  Results are identical. */
+void p3dfft_amd_set_flat_order(int mode);
 /* 1 (default): a forward without Y/Z cuts whose output does not overlap its
  * input runs X r2c straight into the output and the Y pass in place there
  * (the Z pass then reads lines the Y pass left in the last-level cache);

@@ -741,20 +741,25 @@ static void measure_plans(State& S) {
     rep += "n=" + std::to_string(n) + ": " + len_describe(n) + " (" + std::to_string(cands.size()) +
            " candidates); ";
   }
-  // schedule switches: tile order of consecutive passes, X r2c into the output
+  // schedule switches: tile order of consecutive passes, X r2c into the
+  // output, tile order of the Z pass (XCD-aware or dispatch order)
   double best = 1e30;
-  int balt = 1, bxin = 1;
+  int balt = 1, bxin = 1, bflat = 0;
   for (int alt = 1; alt >= 0; alt--)
-    for (int xin = 1; xin >= 0; xin--) {
-      set_tile_order(alt != 0);
-      g_x_into_out = xin != 0;
-      const double t = time_pair(S, rin, cspec, rout, s);
-      if (t < best) { best = t; balt = alt; bxin = xin; }
-    }
+    for (int xin = 1; xin >= 0; xin--)
+      for (int fo = 0; fo <= 1; fo++) {
+        set_tile_order(alt != 0);
+        g_x_into_out = xin != 0;
+        set_flat_order(fo);
+        const double t = time_pair(S, rin, cspec, rout, s);
+        if (t < best) { best = t; balt = alt; bxin = xin; bflat = fo; }
+      }
   set_tile_order(balt != 0);
   g_x_into_out = bxin != 0;
+  set_flat_order(bflat);
   rep += std::string("tile order ") + (balt ? "alternating" : "fixed") + "; X r2c " +
-         (bxin ? "into the output" : "into the work buffer") + "; fwd+bwd " + std::to_string(best) + " ms";
+         (bxin ? "into the output" : "into the work buffer") + "; Z tiles " +
+         (bflat ? "in dispatch order" : "XCD-aware") + "; fwd+bwd " + std::to_string(best) + " ms";
   g_plan_report = rep;
   check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
   (void)hipFree(rin);
@@ -1228,6 +1233,7 @@ int p3dfft_amd_precision(void) { return (int)sizeof(real_t); }
 void p3dfft_amd_set_specialised(int on) { set_specialised(on != 0); }
 void p3dfft_amd_set_tile_order(int alternate) { set_tile_order(alternate != 0); }
 void p3dfft_amd_set_wide_offsets(int on) { set_wide_offsets(on != 0); }
+void p3dfft_amd_set_flat_order(int mode) { set_flat_order(mode); }
 void p3dfft_amd_set_x_into_out(int on) { g_x_into_out = (on != 0); }
 
 // 0 (default): the default plan of every length; 1: from the next setup on,
@@ -1262,6 +1268,7 @@ int p3dfft_amd_reset_plans(void) {
   try {
     free_lens();
     set_tile_order(true);
+    set_flat_order(2);
     g_x_into_out = true;
     g_plan_report.clear();
     return 0;

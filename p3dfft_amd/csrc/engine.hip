@@ -45,9 +45,10 @@ __device__ __forceinline__ int xcd_tile(int bid, int nwg) {
   return x * q + (x < rm ? x : rm) + slot;
 }
 
-// tile of this workgroup: XCD-aware, optionally walking from the last tile
-__device__ __forceinline__ int tile_of(int rev) {
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+// tile of this workgroup: XCD-aware (xcd != 0) or in dispatch order,
+// optionally walking from the last tile
+__device__ __forceinline__ int tile_of(int rev, int xcd = 1) {
+  const int t = xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   return rev ? (int)gridDim.x - 1 - t : t;
 }
 
@@ -287,6 +288,12 @@ struct ColArgs {
   const void* tw;
   int tw_lds, tw_off;
   int rev;  // walk the tiles from the last one
+  // XCD-aware tile order (tiles that share cache lines at their edges run in
+  // one XCD's L2); 0: dispatch order, for views whose tiles never share a
+  // line (one outer index: every tile starts at a multiple of TX elements
+  // of one contiguous inner axis), where the concurrent tiles then cover one
+  // contiguous range of each row
+  int xcd;
   const int2* ltab;  // table-addressed rows (see ColIO)
   const int2* stab;
   int tab_off;       // byte offset of the LDS copy of the row tables
@@ -561,7 +568,7 @@ __global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
   // the twiddle staging is completed by the barrier at the end of the first stage
   const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
   const long long ncol = (long long)a.n_inner * a.n_outer;
-  const long long c0 = (long long)tile_of(a.rev) * TX;
+  const long long c0 = (long long)tile_of(a.rev, a.xcd) * TX;
   const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
   const ColIO io = col_io<T, TX>(a, c0, nb);
   if constexpr (std::is_same<Off, TabRows>::value)
@@ -659,7 +666,7 @@ template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL>
 __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
   cpx<T>* lds = (cpx<T>*)g_smem;
   const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
-  const int t = tile_of(a.rev);
+  const int t = tile_of(a.rev, a.xcd);
   const int o = t / a.tiles;
   const int ti = t - o * a.tiles;
   const long long i0 = (long long)ti * TX;
@@ -733,7 +740,7 @@ __global__ __launch_bounds__(NT, W) void k_zsolve_ct(ColArgs a, double scale, in
   cpx<T>* lds = (cpx<T>*)g_smem;
   const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
   const long long ncol = a.n_inner;
-  const long long c0 = (long long)tile_of(a.rev) * TX;
+  const long long c0 = (long long)tile_of(a.rev, a.xcd) * TX;
   const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
   const ColIO io = col_io<T, TX>(a, c0, nb);
   SolveCtx sc;
@@ -1486,8 +1493,31 @@ static void prep_kernels(const Len1D& L) {
 
 static bool g_use_ct = true;
 static bool g_alt_order = true;
+static int g_flat_order = 2;  // 0 XCD-aware, 1 dispatch order, 2 by shape
 static bool g_wide_offsets = false;
 void set_wide_offsets(bool on) { g_wide_offsets = on; }
+void set_flat_order(int mode) { g_flat_order = mode < 0 || mode > 2 ? 2 : mode; }
+int flat_order() { return g_flat_order; }
+// Order of the tiles of a column pass over one contiguous inner axis.  By
+// shape: dispatch order when the row stride (one plane of the view) lies
+// within 1/128 above 512 KiB, 1 MiB or 2 MiB.  With the XCD-aware order the
+// 8 XCDs' concurrent tiles sit in 8 regions of each row, and rows that
+// start a near power of two apart put those regions on few DRAM channels;
+// in dispatch order the tiles running at once cover one contiguous range of
+// each row.  Measured on MI355X (Z fwd / bwd, us, XCD-aware -> dispatch):
+// FP32 512^3 248/213 -> 210/189, 512x256x512 113/114 -> 104/101,
+// 1024x512x128 99/96 -> 93/87; FP64 512^3 448/419 -> 441/397,
+// 512x256x512 249/206 -> 212/186, 256x512x512 362/319 -> 335/286; strides
+// away from those powers lose with dispatch order (FP64 384^3 176/175 ->
+// 211/198, 128x128x1024 54/52 -> 76/76), as do FP32 256x512x512 (+4%) and
+// FP64 1024x512x128 (+10%, a 4 MiB stride).
+static bool flat_dispatch_order(const ColSpec& c) {
+  if (g_flat_order != 2) return g_flat_order == 1;
+  const long long stride = std::max(c.sl_in, c.sl_out) * (long long)sizeof(cplx_t);
+  for (long long p = 1LL << 19; p <= 1LL << 21; p <<= 1)
+    if (stride >= p && stride - p <= p / 128) return true;
+  return false;
+}
 void set_tile_order(bool alternate) { g_alt_order = alternate; }
 bool tile_order_alternate() { return g_alt_order; }
 void set_specialised(bool on) { g_use_ct = on; }
@@ -1650,6 +1680,7 @@ bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStr
   a.kind = COL_C2C;
   a.ltab = ltab; a.stab = stab;
   a.rev = c.rev ? 1 : 0;
+  a.xcd = 1;  // table-addressed rows: tiles share lines across peer blocks
   const int TX = ct->TX;
   a.TX = TX;
   a.dTX = make_fdiv(TX);
@@ -1675,6 +1706,13 @@ bool launch_col(const ColSpec& c, hipStream_t st) {
   a.n_inner = c.n_inner; a.n_outer = c.n_outer;
   a.kind = c.kind;
   a.rev = c.rev ? 1 : 0;
+  // one contiguous inner axis (the Z pass over the flattened (x, y) index):
+  // tiles never share a line, so they may also run in dispatch order, which
+  // lets the concurrently running tiles cover one contiguous range of each
+  // row (see set_flat_order); views whose tiles share lines at their edges
+  // (the Y pass over rows of nx/2+1 elements) keep the XCD-aware order
+  const bool flat = c.n_outer == 1 && c.si_in == 1 && c.si_out == 1;
+  a.xcd = (flat && flat_dispatch_order(c)) ? 0 : 1;
   a.nld = c.nld; a.ld_lo = c.ld_lo; a.ld_hi = c.ld_hi; a.ld_shift = c.ld_shift;
   a.nst = c.nst; a.st_lo = c.st_lo; a.st_shift = c.st_shift;
   int nfft;

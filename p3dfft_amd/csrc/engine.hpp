@@ -114,6 +114,10 @@ void set_tile_order(bool alternate);
 // ones fit (views larger than 4 GiB always use 64-bit ones).
 void set_wide_offsets(bool on);
 bool tile_order_alternate();
+// Tiles of a column pass over one contiguous inner axis (the Z pass): 0 in
+// the XCD-aware order, 1 in dispatch order, 2 (default) chosen by shape.
+void set_flat_order(int mode);
+int flat_order();
 
 // Fused Z-direction spectral solve on a spectrum (nxh, ny, nz) in place:
 // forward FFT along z, multiply by -scale/|k|^2 (k=0 -> 0), inverse FFT.

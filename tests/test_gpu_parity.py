@@ -474,6 +474,34 @@ def test_tile_order_bit_identical(precision):
 
 
 @pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape,ops", [((64, 48, 40), ("fft", "tff")), ((254, 512, 16), ("fft", "tff")),
+                                       ((254, 512, 17), ("ffc", "cff"))])
+def test_flat_order_bit_identical(precision, shape, ops):
+    """The Z pass's tiles in XCD-aware order (0), dispatch order (1) or the
+    order chosen by shape (2) change only the schedule: identical bits.  At
+    254 x 512 the Z view's rows are 128*512 complex apart, 512 KiB (FP32) or
+    1 MiB (FP64), where the shape rule picks dispatch order."""
+    nx, ny, nz = shape
+    f = np.random.default_rng(31).standard_normal((nz, ny, nx))
+    res = []
+    for mode in (0, 1, 2):
+        P = p3dfft_amd.P3DFFT(precision)
+        P.lib.p3dfft_amd_set_flat_order(mode)
+        P.setup((1, 1), nx, ny, nz)
+        try:
+            s = P.ftran_r2c(f, ops[0])
+            res.append((s, P.btran_c2r(s, ops[1])))
+        finally:
+            P.clean()
+            P.lib.p3dfft_amd_set_flat_order(2)
+    for s, b in res[1:]:
+        assert np.array_equal(res[0][0], s) and np.array_equal(res[0][1], b)
+    scale = float(nx * ny * (2 * (nz - 1) if ops[0][2] == "c" else nz))
+    tol = 1e-4 if precision == 4 else 1e-11
+    assert np.abs(res[0][1] / scale - f).max() < tol * np.abs(f).max()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
 @pytest.mark.parametrize("shape,ops", [((64, 48, 40), ("fft", "tff")), ((36, 20, 30), ("fft", "tff"))])
 def test_wide_offsets_bit_identical(precision, shape, ops):
     """The 64-bit-offset column kernels (the path of views over 4 GiB) give
