@@ -1146,7 +1146,7 @@ void free_lens() {
 // and the workgroup sizes the kernels are built for.
 static const int kEPT = 16;
 static const int kNTs[] = {64, 128, 256, 512, 1024};
-static const size_t kLdsMax = 160 * 1024;
+static constexpr size_t kLdsMax = 160 * 1024;
 // data bytes per tile the tile-shape heuristics aim for (two tiles per CU)
 static const size_t kTileBytes = 64 * 1024;
 
@@ -1306,6 +1306,7 @@ struct CtInfo {
   int TX, NTC, BX, NTX, nr;
   int radix[kMaxStages];
   bool twl_col, twl_x;  // twiddles staged in LDS (column / X kernels)
+  bool twl_tab;         // the same for the table-addressed column kernel
   long long twb;        // twiddle table bytes
   void (*col)(bool, const ColArgs&, unsigned, size_t, hipStream_t);
   void (*colio)(bool, bool, bool, const ColArgs&, unsigned, size_t, hipStream_t);
@@ -1330,7 +1331,14 @@ struct CtLen {
   static constexpr long long TWB = (long long)TwCount<Rs...>::value() * sizeof(Tw);
   static constexpr long long DC = round16((long long)TX * N * sizeof(cplx_t));
   static constexpr long long DX = round16((long long)(BX + 1) * N * sizeof(cplx_t));
-  static constexpr bool TWLC = TWB > 0 && DC + TWB <= kCtLdsTw;
+  // twiddles in LDS behind the column tile: when both fit in kCtLdsTw (two
+  // workgroups per CU), or when the tile alone takes more than that (one
+  // workgroup per CU either way) and both fit in the LDS
+  static constexpr bool TWLC =
+      TWB > 0 && (DC + TWB <= kCtLdsTw || (DC > kCtLdsTw && DC + TWB <= (long long)kLdsMax));
+  // the table-addressed column kernel also keeps a table of N rows in LDS
+  // (read from global memory instead, the tables cost more than the twiddles)
+  static constexpr bool TWLT = TWLC && round16(DC + TWB) + N * (long long)sizeof(int2) <= (long long)kLdsMax;
   static constexpr bool TWLX = TWB > 0 && DX + TWB <= kCtLdsTw;
   // waves per SIMD the LDS footprint allows: the register budget of each kernel
   static constexpr int WC = ct_waves(DC + (TWLC ? TWB : 0), NTC);
@@ -1353,8 +1361,8 @@ struct CtLen {
     typedef unsigned U32;
     typedef unsigned long long U64;
     if (a.ltab) {
-      if (fwd) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLC, false, TabRows>, g, NTC, bytes, s, a);
-      else go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, false, TabRows>, g, NTC, bytes, s, a);
+      if (fwd) go(k_colio_ct<real_t, true, TX, NTC, RL, WC, TWLT, false, TabRows>, g, NTC, bytes, s, a);
+      else go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLT, false, TabRows>, g, NTC, bytes, s, a);
       return;
     }
     if (fwd) {
@@ -1384,7 +1392,7 @@ struct CtLen {
   template <bool FWD>
   static void prep_dir() {
     allow_big_lds(k_col_ct<real_t, FWD, TX, NTC, RL, WC, TWLC>);
-    allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, TabRows>);
+    allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLT, false, TabRows>);
     allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, true, unsigned>);
     allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, unsigned long long>);
     allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, unsigned>);
@@ -1421,7 +1429,7 @@ struct CtLen {
     const int rs[] = {Rs...};
     c.nr = (int)sizeof...(Rs);
     for (int i = 0; i < c.nr; i++) c.radix[i] = rs[i];
-    c.twl_col = TWLC; c.twl_x = TWLX; c.twb = TWB;
+    c.twl_col = TWLC; c.twl_x = TWLX; c.twl_tab = TWLT; c.twb = TWB;
     c.col = &col; c.colio = &colio; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve; c.prep = &prep;
     c.xgrp = &xgrp; c.cgrp = &cgrp; c.zgrp = &zgrp;
     return c;
@@ -1687,7 +1695,7 @@ bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStr
   a.tiles = (c.n_inner + TX - 1) / TX;
   a.L = L.d;
   a.tw = L.tw;
-  size_t bytes = ct_lds((size_t)TX * c.n * sizeof(cplx_t), ct->twl_col, *ct, &a.tw_lds, &a.tw_off);
+  size_t bytes = ct_lds((size_t)TX * c.n * sizeof(cplx_t), ct->twl_tab, *ct, &a.tw_lds, &a.tw_off);
   a.tab_off = (int)round16((long long)bytes);
   bytes = (size_t)a.tab_off + (size_t)c.n * sizeof(int2);
   if (bytes > kLdsMax) return false;

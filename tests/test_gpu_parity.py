@@ -835,6 +835,22 @@ def test_nz_8192_c2c_thin_grid(precision):
 
 
 @pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape", [(126, 40, 2048), (126, 2048, 40)])
+def test_length_2048_columns_against_oracle(precision, shape):
+    """Length-2048 column passes over more tiles than the GPU has CUs (the
+    share-mode Y and Z lengths): the Z pass at (126, 40, 2048) and the Y pass
+    at (126, 2048, 40) against the oracle, and the round trip."""
+    nx, ny, nz = shape
+    a = np.random.default_rng(2048 + ny).standard_normal((nz, ny, nx))
+    ref = oplan(nx, ny, nz).ftran_r2c(a)
+    with lib_setup(precision, nx, ny, nz) as P:
+        s = P.ftran_r2c(a, "fft")
+        b = P.btran_c2r(s, "tff")
+    assert rel(s, ref) < TOL[precision]
+    assert np.abs(b / (nx * ny * nz) - a).max() < PREC[precision] * 100 * np.abs(a).max()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
 @pytest.mark.parametrize("shape,nzc,stride1", [((16, 12, 33), None, False), ((24, 20, 129), None, True),
                                                ((20, 16, 65), 40, False), ((8, 6, 257), None, False)])
 def test_cheby_recurrence_in_the_dct_pass(precision, shape, nzc, stride1):
