@@ -267,6 +267,11 @@ void p3dfft_amd_set_wide_offsets(int on);
  * order (the tiles running at once cover one contiguous range of each row),
  * 2 (default) chosen by the view's shape.  Results are identical. */
 void p3dfft_amd_set_flat_order(int mode);
+/* X and Y of each z-plane in one workgroup (one launch and one HBM round trip
+ * fewer per direction) for the specialised (nx, ny) pairs 32/64/128 without
+ * X/Y cuts: 0 never, 1 wherever they apply, 2 (default) on small grids only.
+ * Their time is reported in timer slot 5 (forward) / 12 (backward). */
+void p3dfft_amd_set_plane_kernels(int mode);
 /* 1 (default): a forward without Y/Z cuts whose output does not overlap its
  * input runs X r2c straight into the output and the Y pass in place there
  * (the Z pass then reads lines the Y pass left in the last-level cache);

@@ -240,7 +240,15 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
   // there, so the Y pass reads (and leaves in the last-level cache) the lines
   // the Z pass reads next
   const bool xin = g_x_into_out && direct && dny == 0 && !alias && zc == nz;
-  if (xin) {
+  // small grids: X and Y of each z-plane in one workgroup (one launch and one
+  // HBM round trip fewer); the fused pass is timed in slot 5 (X)
+  bool plane = false;
+  if (direct && dny == 0 && !alias && zc == nz && nxhpc == nx / 2 + 1 && nx % 2 == 0 && ny % 2 == 0) {
+    tstart(S, 5, s);
+    plane = launch_xy_plane(true, in, out, nx, ny, nz, s);
+    tstop(S, 5, s);
+  }
+  if (xin && !plane) {
     tstart(S, 5, s);
     launch_x_r2c(in, out, (long long)ny * nz, nx, nxhpc, nx, nxhpc, s);
     tstop(S, 5, s);
@@ -256,7 +264,7 @@ bool forward_dev(State& S, const real_t* in, cplx_t* out, const unsigned char* o
     launch_col(y, s);
     tstop(S, 7, s);
   }
-  for (int z0 = 0; z0 < (xin ? 0 : nz); z0 += zc) {
+  for (int z0 = 0; z0 < (xin || plane ? 0 : nz); z0 += zc) {
     const int nzk = std::min(zc, nz - z0);
     tstart(S, 5, s);
     launch_x_r2c(in + (size_t)z0 * ny * nx, S.w1, (long long)ny * nzk, nx, nxhpc, nx, nxhpc, s);
@@ -365,6 +373,14 @@ bool backward_dev(State& S, cplx_t* in, real_t* out, const unsigned char* op, hi
     const char* zb = (const char*)zdst;
     const char* ob = (const char*)out;
     yip = !(ob < zb + (size_t)nxy * nz * sizeof(cplx_t) && zb < ob + (size_t)nx * ny * nz * sizeof(real_t));
+  }
+  // small grids: Y and X c2r of each z-plane in one workgroup, timed in slot
+  // 12 (X)
+  if (yip && zc == nz && nxhpc == nx / 2 + 1 && nx % 2 == 0 && ny % 2 == 0) {
+    tstart(S, 12, s);
+    const bool plane = launch_xy_plane(false, zdst, out, nx, ny, nz, s);
+    tstop(S, 12, s);
+    if (plane) return true;
   }
   for (int z0 = 0; z0 < nz; z0 += zc) {
     const int nzk = std::min(zc, nz - z0);
@@ -1234,6 +1250,7 @@ void p3dfft_amd_set_specialised(int on) { set_specialised(on != 0); }
 void p3dfft_amd_set_tile_order(int alternate) { set_tile_order(alternate != 0); }
 void p3dfft_amd_set_wide_offsets(int on) { set_wide_offsets(on != 0); }
 void p3dfft_amd_set_flat_order(int mode) { set_flat_order(mode); }
+void p3dfft_amd_set_plane_kernels(int mode) { set_plane_kernels(mode); }
 void p3dfft_amd_set_x_into_out(int on) { g_x_into_out = (on != 0); }
 
 // 0 (default): the default plan of every length; 1: from the next setup on,

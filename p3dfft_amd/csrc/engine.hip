@@ -529,6 +529,114 @@ __global__ __launch_bounds__(NT, W) void k_x_c2r_ct(XArgs a) {
   x_c2r_ct_tile<T, B, NT, RL, XT>(lds, tw, a, pair0, pair0, nr);
 }
 
+// ---------------------------------------------------------------------------
+// Plane kernels for small grids: the X and Y transforms of one z-plane in one
+// workgroup, the plane's half spectrum kept in LDS between them.  On small
+// grids every pass is one dependent launch, and a dependent launch costs
+// more than the plane's data movement, so fusing X and Y saves one launch and
+// one HBM round trip per direction.  NX, NY are compile-time lengths with
+// specialised radix schedules RLX, RLY (the twiddle tables of those lengths).
+// LDS: the X tile (NY/2 row pairs of length NX, element e of pair b at
+// lx[e*(NY/2+1) + b]), then the Y tile (NX/2+1 columns of length NY, element
+// y of column k at ly[y*(NX/2+1) + k]), then both twiddle tables.
+// ---------------------------------------------------------------------------
+struct PlaneArgs {
+  const void* in;
+  void* out;
+  const void* twx;  // twiddles of NX, of NY (the same table when NX == NY)
+  const void* twy;
+  int ntwx, ntwy;   // entries
+};
+
+template <class T, class RLX, class RLY>
+struct PlaneGeom {
+  static constexpr int NX = RL_N<RLX>::value, NY = RL_N<RLY>::value;
+  static constexpr int BX = NY / 2, BPX = BX + 1;  // row pairs, padded
+  static constexpr int H = NX / 2 + 1;             // columns of the half spectrum
+  static constexpr long long X_BYTES = (long long)NX * BPX * sizeof(cpx<T>);
+  static constexpr long long Y_BYTES = (long long)NY * H * sizeof(cpx<T>);
+  static constexpr long long DATA = X_BYTES + Y_BYTES;
+};
+
+template <class T, class RLX, class RLY, int NT>
+__device__ __forceinline__ void plane_twiddles(const PlaneArgs& a, const typename TwT<T>::type** tx,
+                                               const typename TwT<T>::type** ty) {
+  typedef PlaneGeom<T, RLX, RLY> G;
+  typedef typename TwT<T>::type TwL;
+  TwL* t = (TwL*)(g_smem + G::DATA);
+  for (int i = threadIdx.x; i < a.ntwx; i += NT) t[i] = ((const TwL*)a.twx)[i];
+  *tx = t;
+  if (a.twy == a.twx) {
+    *ty = t;
+  } else {
+    for (int i = threadIdx.x; i < a.ntwy; i += NT) t[a.ntwx + i] = ((const TwL*)a.twy)[i];
+    *ty = t + a.ntwx;
+  }
+}
+
+// forward: X r2c (two-for-one, as k_x_r2c_ct) and Y forward of plane
+// blockIdx.x: real rows of NX (row stride NX) -> half-spectrum rows of NX/2+1
+template <class T, class RLX, class RLY, int NT>
+__global__ __launch_bounds__(NT) void k_xy_plane_fwd(PlaneArgs a) {
+  typedef PlaneGeom<T, RLX, RLY> G;
+  constexpr int NX = G::NX, NY = G::NY, BX = G::BX, BPX = G::BPX, H = G::H;
+  cpx<T>* lx = (cpx<T>*)g_smem;
+  cpx<T>* ly = (cpx<T>*)(g_smem + G::X_BYTES);
+  const typename TwT<T>::type *tx, *ty;
+  plane_twiddles<T, RLX, RLY, NT>(a, &tx, &ty);
+  const T* in = (const T*)a.in + (long long)blockIdx.x * NX * NY;
+  for (int q = threadIdx.x; q < BX * NX; q += NT) {
+    const int b = q / NX, j = q % NX;
+    lx[j * BPX + b] = mk<T>(in[2 * b * NX + j], in[(2 * b + 1) * NX + j]);
+  }
+  __syncthreads();
+  run_fft_ct<T, true, BX, BPX, NT>(lx, tx, RLX{});
+  const T h = (T)0.5;
+  for (int q = threadIdx.x; q < BX * H; q += NT) {
+    const int b = q / H, k = q % H;
+    const cpx<T> zk = lx[k * BPX + b], zn = lx[(k == 0 ? 0 : NX - k) * BPX + b];
+    ly[(2 * b) * H + k] = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h);
+    ly[(2 * b + 1) * H + k] = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
+  }
+  __syncthreads();
+  run_fft_ct<T, true, H, H, NT>(ly, ty, RLY{});
+  cpx<T>* out = (cpx<T>*)a.out + (long long)blockIdx.x * H * NY;
+  for (int q = threadIdx.x; q < NY * H; q += NT) out[q] = ly[q];
+}
+
+// backward: Y backward and X c2r (as k_x_c2r_ct: Im of the DC and Nyquist
+// modes ignored) of plane blockIdx.x
+template <class T, class RLX, class RLY, int NT>
+__global__ __launch_bounds__(NT) void k_yx_plane_bwd(PlaneArgs a) {
+  typedef PlaneGeom<T, RLX, RLY> G;
+  constexpr int NX = G::NX, NY = G::NY, BX = G::BX, BPX = G::BPX, H = G::H;
+  cpx<T>* lx = (cpx<T>*)g_smem;
+  cpx<T>* ly = (cpx<T>*)(g_smem + G::X_BYTES);
+  const typename TwT<T>::type *tx, *ty;
+  plane_twiddles<T, RLX, RLY, NT>(a, &tx, &ty);
+  const cpx<T>* in = (const cpx<T>*)a.in + (long long)blockIdx.x * H * NY;
+  for (int q = threadIdx.x; q < NY * H; q += NT) ly[q] = in[q];
+  __syncthreads();
+  run_fft_ct<T, false, H, H, NT>(ly, ty, RLY{});
+  for (int q = threadIdx.x; q < BX * H; q += NT) {
+    const int b = q / H, k = q % H;
+    cpx<T> A = ly[(2 * b) * H + k], Bv = ly[(2 * b + 1) * H + k];
+    if (k == 0 || 2 * k == NX) { A.y = 0; Bv.y = 0; }
+    // at k = 0 and k = NX/2 both forms are the same value
+    lx[k * BPX + b] = mk<T>(A.x - Bv.y, A.y + Bv.x);
+    lx[(k == 0 ? 0 : NX - k) * BPX + b] = mk<T>(A.x + Bv.y, Bv.x - A.y);
+  }
+  __syncthreads();
+  run_fft_ct<T, false, BX, BPX, NT>(lx, tx, RLX{});
+  T* out = (T*)a.out + (long long)blockIdx.x * NX * NY;
+  for (int q = threadIdx.x; q < BX * NX; q += NT) {
+    const int b = q / NX, j = q % NX;
+    const cpx<T> z = lx[j * BPX + b];
+    out[2 * b * NX + j] = z.x;
+    out[(2 * b + 1) * NX + j] = z.y;
+  }
+}
+
 // Tile of B consecutive columns c0 .. c0+B-1 of the flattened (outer, inner)
 // column index c = o * n_inner + i, element (o, i, row e) at
 // o*so + i*si + e*sl.  The tile base is column c0's address; lane offsets are
@@ -1660,6 +1768,85 @@ void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_r
   x_setup(a, L, nrows, nx, &bytes, &nt, &nwg);
   dispatch_nt<GoXc2r>(nt, a, nwg, bytes, st);
   check_hip(hipGetLastError(), "k_x_c2r launch");
+}
+
+// Plane kernels (k_xy_plane_fwd / k_yx_plane_bwd) for the (nx, ny) pairs
+// instantiated below; mode 0 off, 1 wherever they apply, 2 (default) on the
+// grids where they measured faster: up to 2^22 points, and at least 64 planes
+// unless the planes are 64 x 64 or smaller (one workgroup per plane).
+// Measured on MI355X, eager fwd+bwd, separate passes -> plane kernels (us):
+// FP64 32^3 22.5 -> 18.2, 64^3 24.3 -> 22.4, 64x64x256 47.7 -> 36.4,
+// 128x64x64 32.8 -> 30.9; FP32 64x64x256 33.0 -> 26.4, 128^3 47.8 -> 40.4,
+// 128x128x64 32.5 -> 32.1, but 128x128x32 25.9 -> 28.1.
+static int g_plane_mode = 2;
+void set_plane_kernels(int mode) { g_plane_mode = mode < 0 || mode > 2 ? 2 : mode; }
+
+template <class RLX, class RLY>
+struct PlaneLen {
+  typedef PlaneGeom<real_t, RLX, RLY> G;
+  // one butterfly of the first X stage per thread
+  static constexpr int NT0 = G::BX * G::NX / 8;
+  static constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : (NT0 + 63) / 64 * 64);
+  static void prep() {
+    allow_big_lds(k_xy_plane_fwd<real_t, RLX, RLY, NT>);
+    allow_big_lds(k_yx_plane_bwd<real_t, RLX, RLY, NT>);
+  }
+  static void go(bool fwd, const PlaneArgs& a, int nz, size_t bytes, hipStream_t s) {
+    if (fwd) k_xy_plane_fwd<real_t, RLX, RLY, NT><<<nz, NT, bytes, s>>>(a);
+    else k_yx_plane_bwd<real_t, RLX, RLY, NT><<<nz, NT, bytes, s>>>(a);
+  }
+};
+
+struct PlaneInfo {
+  long long data;  // LDS bytes of the two tiles
+  void (*prep)();
+  void (*go)(bool, const PlaneArgs&, int, size_t, hipStream_t);
+};
+template <class RLX, class RLY>
+static PlaneInfo plane_info() {
+  typedef PlaneLen<RLX, RLY> P;
+  return PlaneInfo{P::G::DATA, &P::prep, &P::go};
+}
+
+static const PlaneInfo* find_plane(int nx, int ny) {
+  typedef RList<8, 4> R32;
+  typedef RList<8, 8> R64;
+  typedef RList<8, 4, 4> R128;
+  static const std::map<std::pair<int, int>, PlaneInfo> tab = {
+      {{32, 32}, plane_info<R32, R32>()},   {{64, 64}, plane_info<R64, R64>()},
+      {{32, 64}, plane_info<R32, R64>()},   {{64, 32}, plane_info<R64, R32>()},
+      {{128, 64}, plane_info<R128, R64>()}, {{64, 128}, plane_info<R64, R128>()},
+      {{128, 128}, plane_info<R128, R128>()}};
+  auto it = tab.find({nx, ny});
+  return it == tab.end() ? nullptr : &it->second;
+}
+
+bool launch_xy_plane(bool fwd, const void* in, void* out, int nx, int ny, int nz, hipStream_t st) {
+  if (g_plane_mode == 0 || !g_use_ct || nz <= 0) return false;
+  if (g_plane_mode == 2 &&
+      ((long long)nx * ny * nz > (1LL << 22) || (nz < 64 && (long long)nx * ny > 64 * 64)))
+    return false;
+  const PlaneInfo* p = find_plane(nx, ny);
+  if (!p) return false;
+  const Len1D& Lx = get_len(nx);
+  const Len1D& Ly = get_len(ny);
+  // the kernels' radix schedules must be the plans' (the twiddle layout)
+  if (!find_ct(Lx) || !find_ct(Ly)) return false;
+  PlaneArgs a;
+  a.in = in; a.out = out;
+  a.twx = Lx.tw; a.twy = Ly.tw;
+  a.ntwx = Lx.d.twcount; a.ntwy = nx == ny ? 0 : Ly.d.twcount;
+  const size_t bytes = (size_t)p->data + (size_t)(a.ntwx + a.ntwy) * sizeof(Tw);
+  if (bytes > kLdsMax) return false;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    for (int x : {32, 64, 128})
+      for (int y : {32, 64, 128})
+        if (const PlaneInfo* q = find_plane(x, y)) q->prep();
+  });
+  p->go(fwd, a, nz, bytes, st);
+  check_hip(hipGetLastError(), fwd ? "k_xy_plane_fwd launch" : "k_yx_plane_bwd launch");
+  return true;
 }
 
 // The byte offsets of a column tile relative to its first column stay below

@@ -75,6 +75,15 @@ bool launch_x_r2c_tab(const real_t* in, cplx_t* out, long long nrows, int nx, in
 bool launch_x_c2r_tab(const cplx_t* in, real_t* out, long long nrows, int nx, int nxi, const int2* xt,
                       hipStream_t st);
 
+// X and Y transforms of whole z-planes in one launch (one workgroup per
+// plane, the plane kept in LDS), for small grids of the (nx, ny) pairs that
+// have such kernels: forward real (nx, ny, nz) -> half spectrum (nx/2+1, ny,
+// nz), backward the reverse (Im of the DC and Nyquist modes ignored).  False
+// when the grid does not qualify (the caller runs the separate passes).
+bool launch_xy_plane(bool fwd, const void* in, void* out, int nx, int ny, int nz, hipStream_t st);
+// 0: never; 1: wherever they apply; 2 (default): on small grids only.
+void set_plane_kernels(int mode);
+
 // Column transform along a strided axis of a 3D view [outer][len][inner].
 enum ColKind { COL_C2C = 0, COL_DCT1 = 1, COL_DST1 = 2, COL_COPY = 3 };
 struct ColSpec {

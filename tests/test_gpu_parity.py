@@ -334,21 +334,32 @@ def test_error_paths():
     P.clean()
 
 
-def test_timers_accumulate():
+@pytest.mark.parametrize("plane", [0, 1])
+def test_timers_accumulate(plane):
+    """Stage timers in the reference's slots (5 X, 7 Y, 8 Z forward; 9 Z,
+    10 Y, 12 X backward); with the plane kernels (X and Y of a z-plane in
+    one launch) the fused pass is timed in the X slot and the Y slots stay 0."""
     nx = ny = nz = 32
     a = np.random.default_rng(0).random((nz, ny, nx))
-    with lib_setup(8, nx, ny, nz) as P:
-        P.set_timers()
-        assert P.get_timers() == [0.0] * 12
-        s = P.ftran_r2c(a)
-        P.btran_c2r(s)
-        t = P.get_timers()
-        for slot in (5, 7, 8, 9, 10, 12):
-            assert t[slot - 1] > 0.0, slot
-        for slot in (1, 2, 3, 4, 6, 11):
-            assert t[slot - 1] == 0.0
-        P.set_timers()
-        assert P.get_timers() == [0.0] * 12
+    lib = p3dfft_amd.load(8)
+    lib.p3dfft_amd_set_plane_kernels(plane)
+    try:
+        with lib_setup(8, nx, ny, nz) as P:
+            P.set_timers()
+            assert P.get_timers() == [0.0] * 12
+            s = P.ftran_r2c(a)
+            P.btran_c2r(s)
+            t = P.get_timers()
+            busy = (5, 8, 9, 12) if plane else (5, 7, 8, 9, 10, 12)
+            for slot in busy:
+                assert t[slot - 1] > 0.0, slot
+            for slot in range(1, 13):
+                if slot not in busy:
+                    assert t[slot - 1] == 0.0, slot
+            P.set_timers()
+            assert P.get_timers() == [0.0] * 12
+    finally:
+        lib.p3dfft_amd_set_plane_kernels(2)
 
 
 # ---------------------------------------------------------------- C2 / C3 at full size
@@ -832,6 +843,42 @@ def test_nz_8192_c2c_thin_grid(precision):
         with pytest.raises(p3dfft_amd.P3DFFTError, match="needs transform length 16382"):
             P.ftran_r2c(f, "ffc")
 
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape,ops", [((64, 64, 64), ("fft", "tff")), ((32, 32, 20), ("ffc", "cff")),
+                                       ((128, 64, 8), ("fft", "tff")), ((64, 128, 6), ("ffs", "sff")),
+                                       ((32, 64, 12), ("fft", "tff")), ((128, 128, 16), ("fft", "tff"))])
+def test_plane_kernels_against_oracle(precision, shape, ops):
+    """X and Y of each z-plane in one workgroup (plane kernels forced on)
+    against the oracle and against the separate passes (forced off); the
+    forward's Y timer stays zero when the plane kernel ran (its time is in
+    the X slot).  FP64 128 x 128 planes do not fit the LDS and take the
+    separate passes either way."""
+    nx, ny, nz = shape
+    a = np.random.default_rng(nx + ny + nz).standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz)
+    ref_f = o.ftran_r2c(a, ops[0])
+    lib = p3dfft_amd.load(precision)
+    got = []
+    try:
+        for mode in (1, 0):
+            lib.p3dfft_amd_set_plane_kernels(mode)
+            with lib_setup(precision, nx, ny, nz) as P:
+                P.set_timers()
+                s = P.ftran_r2c(a, ops[0])
+                t = P.get_timers()
+                got.append((s, P.btran_c2r(s, ops[1]), t))
+    finally:
+        lib.p3dfft_amd_set_plane_kernels(2)
+    ref_b = o.btran_c2r(got[0][0].astype(np.complex128), ops[1])[0]
+    for s, b, _ in got:
+        assert rel(s, ref_f) < TOL[precision]
+        assert rel(b, ref_b) < TOL[precision]
+    assert rel(got[0][0], got[1][0]) < (1e-14 if precision == 8 else 1e-6)
+    fits = not (precision == 8 and nx == 128 and ny == 128)
+    assert (got[0][2][6] == 0.0) == fits and got[0][2][4] > 0.0
+    assert got[1][2][6] > 0.0
 
 
 @pytest.mark.parametrize("precision", [8, 4])
