@@ -1775,17 +1775,20 @@ void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_r
 // grids where they measured faster: up to 2^22 points, and at least 64 planes
 // unless the planes are 64 x 64 or smaller (one workgroup per plane).
 // Measured on MI355X, eager fwd+bwd, separate passes -> plane kernels (us):
-// FP64 32^3 22.5 -> 18.2, 64^3 24.3 -> 22.4, 64x64x256 47.7 -> 36.4,
-// 128x64x64 32.8 -> 30.9; FP32 64x64x256 33.0 -> 26.4, 128^3 47.8 -> 40.4,
-// 128x128x64 32.5 -> 32.1, but 128x128x32 25.9 -> 28.1.
+// FP64 32^3 27.2 -> 14.0, 64^3 24.8 -> 19.8, 64x64x256 47.6 -> 33.9,
+// 128x64x64 33.2 -> 29.8; FP32 64x64x256 33.5 -> 23.7, 128^3 48.1 -> 40.8,
+// 128x128x64 32.9 -> 32.1, but 128x128x32 26.5 -> 28.2.
 static int g_plane_mode = 2;
 void set_plane_kernels(int mode) { g_plane_mode = mode < 0 || mode > 2 ? 2 : mode; }
 
 template <class RLX, class RLY>
 struct PlaneLen {
   typedef PlaneGeom<real_t, RLX, RLY> G;
-  // one butterfly of the first X stage per thread
-  static constexpr int NT0 = G::BX * G::NX / 8;
+  // four threads per butterfly of the first X stage (up to 1024): the loads,
+  // the separation and the stores spread over all of them, the butterflies
+  // over a quarter (64^3 FP64 fwd+bwd 22.4 -> 19.8 us against one thread
+  // per butterfly)
+  static constexpr int NT0 = G::BX * G::NX / 8 * 4;
   static constexpr int NT = NT0 < 64 ? 64 : (NT0 > 1024 ? 1024 : (NT0 + 63) / 64 * 64);
   static void prep() {
     allow_big_lds(k_xy_plane_fwd<real_t, RLX, RLY, NT>);
