@@ -1594,6 +1594,7 @@ static void prep_generic_nt() {
 // Kernel attributes (the dynamic-LDS limit) are set when a length is first
 // planned, never at launch: launches are then the only device-side work of a
 // transform, which is what stream capture into a graph requires.
+static void prep_plane_kernels(int n);
 static void prep_kernels(const Len1D& L) {
   static bool generic_done = false;
   if (!generic_done) {
@@ -1605,6 +1606,7 @@ static void prep_kernels(const Len1D& L) {
     generic_done = true;
   }
   if (const CtInfo* c = find_ct(L)) c->prep();
+  prep_plane_kernels(L.d.n);
 }
 
 static bool g_use_ct = true;
@@ -1811,6 +1813,19 @@ static PlaneInfo plane_info() {
   return PlaneInfo{P::G::DATA, &P::prep, &P::go};
 }
 
+static const PlaneInfo* find_plane(int nx, int ny);
+// LDS limits of the plane kernels, raised when a length they use is planned
+// (host-side attribute calls outside any stream capture)
+static void prep_plane_kernels(int n) {
+  static std::once_flag once;
+  if (n != 32 && n != 64 && n != 128) return;
+  std::call_once(once, [] {
+    for (int x : {32, 64, 128})
+      for (int y : {32, 64, 128})
+        if (const PlaneInfo* q = find_plane(x, y)) q->prep();
+  });
+}
+
 static const PlaneInfo* find_plane(int nx, int ny) {
   typedef RList<8, 4> R32;
   typedef RList<8, 8> R64;
@@ -1841,12 +1856,6 @@ bool launch_xy_plane(bool fwd, const void* in, void* out, int nx, int ny, int nz
   a.ntwx = Lx.d.twcount; a.ntwy = nx == ny ? 0 : Ly.d.twcount;
   const size_t bytes = (size_t)p->data + (size_t)(a.ntwx + a.ntwy) * sizeof(Tw);
   if (bytes > kLdsMax) return false;
-  static std::once_flag once;
-  std::call_once(once, [] {
-    for (int x : {32, 64, 128})
-      for (int y : {32, 64, 128})
-        if (const PlaneInfo* q = find_plane(x, y)) q->prep();
-  });
   p->go(fwd, a, nz, bytes, st);
   check_hip(hipGetLastError(), fwd ? "k_xy_plane_fwd launch" : "k_yx_plane_bwd launch");
   return true;

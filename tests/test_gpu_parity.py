@@ -671,12 +671,12 @@ def test_c3_power_law_fp32_against_fp64(n):
 
 
 @pytest.mark.parametrize("precision", [4, 8])
-@pytest.mark.parametrize("shape", [(64, 48, 40), (60, 36, 50)])
+@pytest.mark.parametrize("shape", [(64, 48, 40), (60, 36, 50), (64, 64, 64)])
 def test_captured_transforms_replay_bit_exact(precision, shape):
     """ftran_r2c_dev + btran_c2r_dev captured from a stream into a HIP graph
     (kernel launches only: no allocation, attribute call or memset) replay to
     the eager results bit for bit, three times, with eager transforms and
-    uploads between the replays."""
+    uploads between the replays (64^3: the plane kernels)."""
     import ctypes
     nx, ny, nz = shape
     rng = np.random.default_rng(59)
