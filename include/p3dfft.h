@@ -30,7 +30,8 @@ typedef double p3dfft_real;
 /* build/setup.F90:83 (p3dfft_setup_c, BIND(C,NAME='p3dfft_setup')).
  * dims: processor grid, must be {1,1} (one GPU = one rank); comm: ignored;
  * nxc/nyc/nzc: cut (pruned) sizes, 1..n; ow: nonzero allows btran to
- * overwrite its input; memsize: 3 ints out. */
+ * overwrite its input; memsize: 3 ints out.  Axes up to 2^24 points; those
+ * above 8192 run in several passes over a library-owned scratch buffer. */
 void p3dfft_setup(int *dims, int *nx, int *ny, int *nz, int *comm, int *nxc, int *nyc, int *nzc, int *ow,
                   int *memsize);
 

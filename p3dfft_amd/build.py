@@ -21,7 +21,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "bin", "amdflang")
 
-SOURCES = ["engine.hip", "share.hip", "api.cpp", "api_share.cpp", "api_poisson.cpp", "api_procs.cpp", "serve.cpp"]
+SOURCES = ["engine.hip", "engine_long.hip", "share.hip", "api.cpp", "api_share.cpp", "api_poisson.cpp", "api_procs.cpp", "serve.cpp"]
 # packed-FP32 instructions are disabled for all device code
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -179,10 +179,10 @@ void ensure_staging() {
 // extension 2(nz-1) for the DCT-I and the odd one 2(nz+1) for the DST-I.
 bool zop_len_ok(int nz, int kind) {
   const long long m = kind == COL_DCT1 ? 2LL * (nz - 1) : (kind == COL_DST1 ? 2LL * (nz + 1) : (long long)nz);
-  if (m <= max_len()) return true;
+  if (m <= max_len_any()) return true;
   return fatal(std::string(" p3dfft_amd: the ") + (kind == COL_DCT1 ? "DCT-I" : "DST-I") +
                " along z of nz = " + std::to_string(nz) + " needs transform length " + std::to_string(m) +
-               ", above the supported " + std::to_string(max_len()));
+               ", above the supported " + std::to_string(max_len_any()));
 }
 
 static int zkind(char c, bool* none) {
@@ -721,7 +721,10 @@ static void measure_plans(State& S) {
   check_hip(hipMalloc(&rout, nr * sizeof(real_t)), "hipMalloc(measure)");
   check_hip(hipMalloc(&cspec, nc * sizeof(cplx_t)), "hipMalloc(measure)");
   (void)p3dfft_amd_fill_normal(rin, (long long)nr, 12345, (void*)s);
-  std::vector<int> lens = {S.nx, S.ny, S.nz};
+  std::vector<int> lens;
+  // lengths above max_len() keep their multi-pass plans
+  for (int n : {S.nx, S.ny, S.nz})
+    if (n <= max_len()) lens.push_back(n);
   std::sort(lens.begin(), lens.end());
   lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
   std::string rep;
@@ -800,6 +803,7 @@ void release_state() {
   g.nv_alloc = 0;
   g.stage_a = g.stage_b = nullptr;
   g.stage_bytes = 0;
+  free_long_scratch();
   g.set = false;
 }
 
@@ -903,7 +907,7 @@ void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int
   g.stride1 = g.stride1_next;
   compute_geometry(g, dims, 0, 0);
   if (g.stride1) printf(" Using stride-1 layout\n");
-  const int lim = max_len();
+  const int lim = max_len_any();
   // the Z length of a DCT-I / DST-I op is checked when that op is requested
   if (g.nx > lim || g.ny > lim || g.nz > lim) {
     fatal(" p3dfft_amd: grid dimension exceeds the supported transform length " + std::to_string(lim));
@@ -916,9 +920,9 @@ void p3dfft_setup(int* dims, int* nx, int* ny, int* nz, int* comm, int* nxc, int
   // entry points (a failure is reported by those entry points)
   (void)scratch_ptr(false);
   // build the 1D plans (twiddles are generated on the GPU here, once)
-  get_len(g.nx);
-  get_len(g.ny);
-  get_len(g.nz);
+  plan_len(g.nx);
+  plan_len(g.ny);
+  plan_len(g.nz);
   if (g_plan_mode == 1) measure_plans(g);
   g.set = true;
   memsize[0] = g.memsize[0];
@@ -1284,6 +1288,7 @@ int p3dfft_amd_reset_plans(void) {
   }
   try {
     free_lens();
+    free_long();
     set_tile_order(true);
     set_flat_order(2);
     g_x_into_out = true;

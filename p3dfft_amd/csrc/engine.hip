@@ -1693,6 +1693,10 @@ static bool x_setup_ct(XArgs& a, const Len1D& L, const CtInfo& c, long long nrow
 void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_row, long long so_row, int nx,
                   int nxo, hipStream_t st, bool rev) {
   if (nrows <= 0 || nxo <= 0) return;
+  if (nx > max_len()) {
+    launch_x_long(true, in, out, nrows, si_row, so_row, nx, nxo, st);
+    return;
+  }
   const Len1D& L = get_len(nx);
   XArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -1716,6 +1720,7 @@ void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_r
 bool launch_x_r2c_tab(const real_t* in, cplx_t* out, long long nrows, int nx, int nxo, const int2* xt,
                       hipStream_t st) {
   if (nrows <= 0 || nxo <= 0) return true;
+  if (nx > max_len()) return false;
   const Len1D& L = get_len(nx);
   const CtInfo* c = g_use_ct ? find_ct(L) : nullptr;
   if (!c) return false;
@@ -1734,6 +1739,7 @@ bool launch_x_r2c_tab(const real_t* in, cplx_t* out, long long nrows, int nx, in
 bool launch_x_c2r_tab(const cplx_t* in, real_t* out, long long nrows, int nx, int nxi, const int2* xt,
                       hipStream_t st) {
   if (nrows <= 0) return true;
+  if (nx > max_len()) return false;
   const Len1D& L = get_len(nx);
   const CtInfo* c = g_use_ct ? find_ct(L) : nullptr;
   if (!c) return false;
@@ -1752,6 +1758,10 @@ bool launch_x_c2r_tab(const cplx_t* in, real_t* out, long long nrows, int nx, in
 void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_row, long long so_row, int nx,
                   int nxi, hipStream_t st, bool rev) {
   if (nrows <= 0) return;
+  if (nx > max_len()) {
+    launch_x_long(false, in, out, nrows, si_row, so_row, nx, nxi, st);
+    return;
+  }
   const Len1D& L = get_len(nx);
   XArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -1874,7 +1884,7 @@ static bool col_fits32(const ColSpec& c, int TX, int rows) {
 
 bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStream_t st) {
   if (c.n_inner <= 0 || c.n_outer <= 0) return true;
-  if (c.kind != COL_C2C || !g_use_ct) return false;
+  if (c.kind != COL_C2C || !g_use_ct || c.n > max_len()) return false;
   const Len1D& L = get_len(c.n);
   const CtInfo* ct = find_ct(L);
   if (!ct) return false;
@@ -1931,6 +1941,10 @@ bool launch_col(const ColSpec& c, hipStream_t st) {
       if (c.n < 2) throw Error{"DCT-I needs at least 2 points", -1};
       m = 2 * (c.n - 1);
     } else if (c.kind == COL_DST1) m = 2 * (c.n + 1);
+    if (m > max_len()) {
+      launch_col_long(c, m, st);
+      return false;
+    }
     L = &get_len(m);
     nfft = m;
   }

@@ -57,6 +57,16 @@ std::string len_describe(int n);
 
 // Largest transform length the engine runs in one workgroup.
 int max_len();
+// Largest transform length accepted at all: longer ones (and DCT-I / DST-I
+// extensions above max_len()) run in several passes over a scratch buffer
+// (engine_long.hip): four-step, or Bluestein where the length does not split.
+int max_len_any();
+// build the plan of a length of either kind (twiddles, sub-lengths, filters)
+void plan_len(int n);
+// free the multi-pass plans and their scratch (no captured launch may use them)
+void free_long();
+// free only the multi-pass scratch (grown again on the next long transform)
+void free_long_scratch();
 
 // Row transforms along the contiguous axis (two rows per complex FFT).
 //   r2c: in real rows (stride si_row reals), out complex rows (stride so_row),
@@ -108,6 +118,12 @@ struct ColSpec {
 // Returns true when a requested Chebyshev recurrence was applied (false: the
 // caller runs launch_cheby on the output).
 bool launch_col(const ColSpec& c, hipStream_t st);
+// the multi-pass form of launch_col for an FFT length m above max_len()
+void launch_col_long(const ColSpec& c, int m, hipStream_t st);
+// the multi-pass X r2c (r2c) or c2r of rows of length nx above max_len();
+// nk = nxo (r2c) or nxi (c2r)
+void launch_x_long(bool r2c, const void* in, void* out, long long nrows, long long si_row, long long so_row, int nx,
+                   int nk, hipStream_t st);
 // C2C column transform of n_inner x n_outer columns of length n whose rows
 // are table-addressed: row e of column (o, i) at element ltab[e].x + o *
 // ltab[e].y + i of c.in (zero when ltab[e].x < 0), FFT row p stored at

@@ -59,6 +59,32 @@ def test_c2r_drops_imag_of_dc_and_nyquist():
         assert rel(a, np.fft.irfft(Y, n) * n) < 1e-14
 
 
+@pytest.mark.parametrize("n", [16384, 12000, 9000, 8209])
+def test_oracle_long_lengths_against_numpy(n):
+    """The oracle at lengths above 8192 (those the GPU engine runs in several
+    passes: four-step 128x128, 100x120, 90x100, and the prime 8209) against
+    numpy's pocketfft, an independent implementation: complex forward and
+    backward, r2c and c2r."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert rel(O.fft(x, -1), np.fft.fft(x)) < 1e-13
+    assert rel(O.fft(x, +1), np.fft.ifft(x) * n) < 1e-13
+    r = x.real.copy()
+    assert rel(O.r2c(r), np.fft.rfft(r)) < 1e-13
+    X = np.fft.rfft(r)
+    assert rel(O.c2r(X, n), r * n) < 1e-13
+
+
+@pytest.mark.parametrize("n", [4200, 4500, 8192])
+def test_oracle_long_dct1_dst1_against_scipy(n):
+    """DCT-I / DST-I whose extensions (2(n-1), 2(n+1)) exceed 8192 points,
+    against scipy.fft's type-1 transforms (FFTW's REDFT00 / RODFT00)."""
+    import scipy.fft as sf
+    r = np.random.default_rng(n).standard_normal(n)
+    assert rel(O.dct1(r).real, sf.dct(r, type=1)) < 1e-13
+    assert rel(O.dst1(r).real, sf.dst(r, type=1)) < 1e-13
+
+
 @pytest.mark.parametrize("n", [2, 3, 9, 33, 65])
 def test_dct1_dst1_self_inverse(n):
     rng = np.random.default_rng(n)
