@@ -273,6 +273,11 @@ void p3dfft_amd_set_flat_order(int mode);
  * X/Y cuts: 0 never, 1 wherever they apply, 2 (default) on small grids only.
  * Their time is reported in timer slot 5 (forward) / 12 (backward). */
 void p3dfft_amd_set_plane_kernels(int mode);
+/* X rows of 512 points: 1 through the wave kernels (one wavefront per row
+ * pair; butterfly exchanges by cross-lane permlane/DPP/bpermute operations,
+ * no workgroup barrier), 0 (default) through the LDS-tile kernels, which
+ * measured faster.  No reference counterpart; results agree to rounding. */
+void p3dfft_amd_set_x_wave(int on);
 /* 1 (default): a forward without Y/Z cuts whose output does not overlap its
  * input runs X r2c straight into the output and the Y pass in place there
  * (the Z pass then reads lines the Y pass left in the last-level cache);

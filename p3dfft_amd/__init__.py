@@ -89,6 +89,7 @@ def load(precision=8):
     L.p3dfft_amd_set_wide_offsets.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_flat_order.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_plane_kernels.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_set_x_wave.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_x_into_out.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_plan_mode.argtypes = [ctypes.c_int]
     L.p3dfft_amd_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.POINTER(_vp)]
@@ -125,6 +126,8 @@ def load(precision=8):
     L.p3dfft_amd_set_chunk_bytes.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_malloc.argtypes = [ctypes.c_size_t]
     L.p3dfft_amd_malloc.restype = _vp
+    L.p3dfft_amd_fill_normal.argtypes = [_vp, ctypes.c_longlong, ctypes.c_ulonglong, _vp]
+    L.p3dfft_amd_fill_normal.restype = ctypes.c_int
     L.p3dfft_amd_free.argtypes = [_vp]
     for n in ("p3dfft_amd_memcpy_h2d", "p3dfft_amd_memcpy_d2h", "p3dfft_amd_memcpy_d2d"):
         getattr(L, n).argtypes = [_vp, _vp, ctypes.c_size_t]

@@ -530,6 +530,287 @@ __global__ __launch_bounds__(NT, W) void k_x_c2r_ct(XArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Wave kernels for X rows of 512 points: one wavefront per row pair, no LDS
+// tile and no workgroup barrier.  Lane l holds the 8 values l + 64m of the
+// complex row z = x_a + i x_b (m = 0..7, registers).  The length-512 FFT is
+// 512 = 8 x 64 with the 64-point part over the lane index, 64 = 8 x 8:
+//   radix-8 over the registers (stride 64), twiddle W_512^(l k1);
+//   exchange lane bits 3-5 <-> register bits 0-2 (permlane32_swap,
+//   permlane16_swap, DPP row_ror:8), radix-8 over the registers, twiddle
+//   W_64^(l0 k2);
+//   exchange lane bits 0-2 <-> register bits 0-2 (DPP row_ror:4/12 with bank
+//   masks, DPP quad_perm), radix-8 over the registers.
+// Lane s + 8q then holds Z[q + 8s + 64r] in register r; ds_bpermute gives
+// every lane the contiguous run k = l + 64r to store, and the two-for-one
+// separation's partner Z[N-k].  Every exchange is a cross-lane VALU (or
+// bpermute) operation of one wave.
+// ---------------------------------------------------------------------------
+// swap lane bit LB with the register pair (lo, hi): a lane whose bit LB is 0
+// keeps lo and receives its partner's lo as hi; a lane whose bit is 1 keeps
+// hi and receives its partner's hi as lo (partner = lane ^ (1 << LB))
+template <int LB>
+__device__ __forceinline__ void lswap(unsigned& lo, unsigned& hi) {
+  if constexpr (LB == 5) {
+    const auto r = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+    lo = r[0];
+    hi = r[1];
+  } else if constexpr (LB == 4) {
+    const auto r = __builtin_amdgcn_permlane16_swap(lo, hi, false, false);
+    lo = r[0];
+    hi = r[1];
+  } else if constexpr (LB == 3 || LB == 2) {
+    // rows of 16 lanes: banks of 4 lanes; bit 3 clear = banks 0,1, bit 2
+    // clear = banks 0,2; partner by rotation inside the row (row_ror:n gives
+    // lane i the value of lane i - n mod 16)
+    constexpr int up = LB == 3 ? 0x128 : 0x12C;    // row_ror:8 / row_ror:12 (lane i <- i + 2^LB)
+    constexpr int dn = LB == 3 ? 0x128 : 0x124;    // row_ror:8 / row_ror:4 (lane i <- i - 2^LB)
+    constexpr int b0 = LB == 3 ? 0x3 : 0x5, b1 = LB == 3 ? 0xC : 0xA;
+    const unsigned t = hi;
+    hi = (unsigned)__builtin_amdgcn_update_dpp((int)hi, (int)lo, up, 0xF, b0, false);
+    lo = (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)t, dn, 0xF, b1, false);
+  } else {
+    // inside a quad: quad_perm [1,0,3,2] (bit 0) / [2,3,0,1] (bit 1)
+    constexpr int qp = LB == 0 ? 0xB1 : 0x4E;
+    const bool b = (threadIdx.x >> LB) & 1;
+    const unsigned send = b ? lo : hi;
+    const unsigned recv = (unsigned)__builtin_amdgcn_mov_dpp((int)send, qp, 0xF, 0xF, false);
+    if (b) lo = recv;
+    else hi = recv;
+  }
+}
+
+template <int LB>
+__device__ __forceinline__ void lswap(float& lo, float& hi) {
+  unsigned a = __float_as_uint(lo), b = __float_as_uint(hi);
+  lswap<LB>(a, b);
+  lo = __uint_as_float(a);
+  hi = __uint_as_float(b);
+}
+
+template <int LB>
+__device__ __forceinline__ void lswap(double& lo, double& hi) {
+  const unsigned long long a = (unsigned long long)__double_as_longlong(lo);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(hi);
+  unsigned al = (unsigned)a, ah = (unsigned)(a >> 32), bl = (unsigned)b, bh = (unsigned)(b >> 32);
+  lswap<LB>(al, bl);
+  lswap<LB>(ah, bh);
+  lo = __longlong_as_double((long long)(((unsigned long long)ah << 32) | al));
+  hi = __longlong_as_double((long long)(((unsigned long long)bh << 32) | bl));
+}
+
+// transpose lane bit LB with register bit RB of an 8-register array
+template <int LB, int RB, class T>
+__device__ __forceinline__ void lswap_regs(cpx<T>* z) {
+#pragma unroll
+  for (int r = 0; r < 8; r++)
+    if (!(r & (1 << RB))) {
+      lswap<LB>(z[r].x, z[r | (1 << RB)].x);
+      lswap<LB>(z[r].y, z[r | (1 << RB)].y);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T bperm(int lane, T v);
+template <>
+__device__ __forceinline__ float bperm<float>(int lane, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v)));
+}
+template <>
+__device__ __forceinline__ double bperm<double>(int lane, double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(lane << 2, (int)u);
+  const int hi = __builtin_amdgcn_ds_bpermute(lane << 2, (int)(u >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// the 512-point FFT of one wave's row (see above); twf[e] = W_512^e
+template <bool FWD, class T>
+__device__ __forceinline__ void wave_fft512(cpx<T>* z, const typename TwT<T>::type* twf) {
+  const int l = threadIdx.x & 63;
+  dft8<FWD>(z);
+#pragma unroll
+  for (int k = 1; k < 8; k++) z[k] = twmul<FWD>(z[k], twf[l * k]);
+  lswap_regs<3, 0>(z);
+  lswap_regs<4, 1>(z);
+  lswap_regs<5, 2>(z);
+  dft8<FWD>(z);
+  const int l0 = l & 7;
+#pragma unroll
+  for (int k = 1; k < 8; k++) z[k] = twmul<FWD>(z[k], twf[8 * l0 * k]);
+  lswap_regs<0, 0>(z);
+  lswap_regs<1, 1>(z);
+  lswap_regs<2, 2>(z);
+  dft8<FWD>(z);
+}
+
+// X r2c of 512-point rows, one wave per row pair at a time; each workgroup
+// (four waves) stages the twiddle table in LDS once and walks row pairs at a
+// stride of the grid, loading the next pair before transforming the current
+template <class T>
+__global__ __launch_bounds__(256) void k_x_r2c_w512(XArgs a, const typename TwT<T>::type* twg) {
+  typedef typename TwT<T>::type TwL;
+  TwL* twf = (TwL*)g_smem;
+  for (int i = threadIdx.x; i < 512; i += 256) twf[i] = twg[i];
+  __syncthreads();
+  const long long npairs = (a.nrows + 1) / 2;
+  const int l = threadIdx.x & 63;
+  const int tl = (l >> 3) + 8 * (l & 7);
+  const int lp = (64 - l) & 63;
+  const int tp = (lp >> 3) + 8 * (lp & 7);
+  const long long stride = (long long)gridDim.x * 4;
+  long long idx = (long long)tile_of(0) * 4 + (threadIdx.x >> 6);
+  // rows of pair index i (walked from the last pair when a.rev)
+  auto rows = [&](long long i, const T*& ia, const T*& ib, long long& r0, bool& hb) {
+    const long long pair = a.rev ? npairs - 1 - i : i;
+    r0 = 2 * pair;
+    hb = r0 + 1 < a.nrows;
+    ia = (const T*)a.in + r0 * a.si_row;
+    ib = hb ? ia + a.si_row : ia;
+  };
+  if (idx >= npairs) return;
+  const T *ia, *ib;
+  long long r0;
+  bool hb;
+  rows(idx, ia, ib, r0, hb);
+  T xa[8], xb[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) { xa[m] = ia[l + 64 * m]; xb[m] = ib[l + 64 * m]; }
+  for (; idx < npairs; idx += stride) {
+    cpx<T> z[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) z[m] = mk<T>(xa[m], hb ? xb[m] : (T)0);
+    const long long cr0 = r0;
+    const bool chb = hb;
+    if (idx + stride < npairs) {
+      rows(idx + stride, ia, ib, r0, hb);
+#pragma unroll
+      for (int m = 0; m < 8; m++) { xa[m] = ia[l + 64 * m]; xb[m] = ib[l + 64 * m]; }
+    }
+    wave_fft512<true>(z, twf);
+    // lane s + 8q holds Z[q + 8s + 64r].  Lane l stores the modes k = l + 64r
+    // (a contiguous 64-mode run per store): Z[k] from lane (l >> 3) + 8(l & 7)
+    // register r, Z[512 - k] from the lane of natural index (64 - l) & 63
+    // register 7 - r; lane 0's partners Z[64(8 - r)] are its own registers
+    // (8 - r) & 7
+    cpx<T> zk[5], zn[5];
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+      zk[r] = mk<T>(bperm<T>(tl, z[r].x), bperm<T>(tl, z[r].y));
+      zn[r] = mk<T>(bperm<T>(tp, z[7 - r].x), bperm<T>(tp, z[7 - r].y));
+    }
+    if (l == 0) {
+#pragma unroll
+      for (int r = 0; r < 5; r++) zn[r] = z[(8 - r) & 7];
+    }
+    cpx<T>* oa = (cpx<T>*)a.out + cr0 * a.so_row;
+    cpx<T>* ob = oa + a.so_row;
+    const T h = (T)0.5;
+#pragma unroll
+    for (int r = 0; r < 5; r++) {
+      const int k = l + 64 * r;
+      if (k < a.nxo) {
+        const cpx<T> u = zk[r], v = zn[r];
+        oa[k] = mk<T>((u.x + v.x) * h, (u.y - v.y) * h);
+        if (chb) ob[k] = mk<T>((u.y + v.y) * h, (v.x - u.x) * h);
+      }
+    }
+  }
+}
+
+// X c2r of 512-point rows, one wave per row pair at a time (workgroups as
+// in k_x_r2c_w512): Z = A + iB from the stored modes k <= 256 (Im of the DC
+// and Nyquist modes ignored, modes at and above nxi zero), the upper half
+// conj(A[512-k]) + i conj(B[512-k]) taken from the lane holding 512 - k
+template <class T>
+__global__ __launch_bounds__(256) void k_x_c2r_w512(XArgs a, const typename TwT<T>::type* twg) {
+  typedef typename TwT<T>::type TwL;
+  TwL* twf = (TwL*)g_smem;
+  for (int i = threadIdx.x; i < 512; i += 256) twf[i] = twg[i];
+  __syncthreads();
+  const long long npairs = (a.nrows + 1) / 2;
+  const int l = threadIdx.x & 63;
+  const int tl = (l >> 3) + 8 * (l & 7);
+  const int src = (64 - l) & 63;
+  const long long stride = (long long)gridDim.x * 4;
+  long long idx = (long long)tile_of(0) * 4 + (threadIdx.x >> 6);
+  auto rows = [&](long long i, const cpx<T>*& ia, const cpx<T>*& ib, long long& r0, bool& hb) {
+    const long long pair = a.rev ? npairs - 1 - i : i;
+    r0 = 2 * pair;
+    hb = r0 + 1 < a.nrows;
+    ia = (const cpx<T>*)a.in + r0 * a.si_row;
+    ib = hb ? ia + a.si_row : ia;
+  };
+  if (idx >= npairs) return;
+  const cpx<T> *ia, *ib;
+  long long r0;
+  bool hb;
+  rows(idx, ia, ib, r0, hb);
+  // modes k = l + 64m (m < 5, k <= 256); out-of-range lanes load element 0
+  // (always valid) and zero it
+  cpx<T> va[5], vb[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const int k = l + 64 * m;
+    const bool ok = k <= 256 && k < a.nxo;
+    va[m] = ia[ok ? k : 0];
+    vb[m] = ib[ok ? k : 0];
+  }
+  for (; idx < npairs; idx += stride) {
+    cpx<T> z[8], w[4];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const int k = l + 64 * m;
+      const bool ok = k <= 256 && k < a.nxo;
+      cpx<T> A = va[m], B = vb[m];
+      if (!ok) A = mk<T>(0, 0);
+      if (!ok || !hb) B = mk<T>(0, 0);
+      if (k == 0 || k == 256) { A.y = 0; B.y = 0; }
+      if (m < 4) {
+        z[m] = mk<T>(A.x - B.y, A.y + B.x);
+        w[m] = mk<T>(A.x + B.y, B.x - A.y);  // conj(A) + i conj(B)
+      } else {
+        z[4] = mk<T>(A.x - B.y, A.y + B.x);  // lane 0: k = 256
+      }
+    }
+    const long long cr0 = r0;
+    const bool chb = hb;
+    if (idx + stride < npairs) {
+      rows(idx + stride, ia, ib, r0, hb);
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        const int k = l + 64 * m;
+        const bool ok = k <= 256 && k < a.nxo;
+        va[m] = ia[ok ? k : 0];
+        vb[m] = ib[ok ? k : 0];
+      }
+    }
+    // k = l + 64m > 256 (m >= 4, l > 0): from lane 64 - l, register 7 - m;
+    // lane 0: k = 64m, from its own register 8 - m (m = 5..7)
+    cpx<T> up[4];
+#pragma unroll
+    for (int m = 4; m < 8; m++) up[m - 4] = mk<T>(bperm<T>(src, w[7 - m].x), bperm<T>(src, w[7 - m].y));
+    if (l == 0) {
+      up[0] = z[4];
+#pragma unroll
+      for (int m = 5; m < 8; m++) up[m - 4] = w[8 - m];
+    }
+#pragma unroll
+    for (int m = 4; m < 8; m++) z[m] = up[m - 4];
+    wave_fft512<false>(z, twf);
+    // lane s + 8q holds x[q + 8s + 64r]; lane l stores n = l + 64r, taken from
+    // lane (l >> 3) + 8(l & 7) register r (a contiguous run per store)
+    T* oa = (T*)a.out + cr0 * a.so_row;
+    T* ob = oa + a.so_row;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const T ua = bperm<T>(tl, z[r].x), ub = bperm<T>(tl, z[r].y);
+      oa[l + 64 * r] = ua;
+      if (chb) ob[l + 64 * r] = ub;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Plane kernels for small grids: the X and Y transforms of one z-plane in one
 // workgroup, the plane's half spectrum kept in LDS between them.  On small
 // grids every pass is one dependent launch, and a dependent launch costs
@@ -1200,6 +1481,19 @@ static Len1D build_len(int n, const std::vector<int>& f) {
   if (L.has_generic)
     for (int e = 0; e < n; e++) idx.push_back(make_int2(e, n));
   L.d.twcount = (int)idx.size();
+  if (n == 512) {
+    // the wave kernels' table W_512^e
+    std::vector<int2> fidx(n);
+    for (int e = 0; e < n; e++) fidx[e] = make_int2(e, n);
+    int2* didx = nullptr;
+    check_hip(hipMalloc(&didx, n * sizeof(int2)), "hipMalloc(twiddle index)");
+    check_hip(hipMemcpy(didx, fidx.data(), n * sizeof(int2), hipMemcpyHostToDevice), "hipMemcpy");
+    check_hip(hipMalloc(&L.twf, n * sizeof(Tw)), "hipMalloc(twiddles)");
+    k_gen_tw<Tw><<<(n + 255) / 256, 256>>>(didx, n, L.twf);
+    check_hip(hipGetLastError(), "k_gen_tw launch");
+    check_hip(hipDeviceSynchronize(), "k_gen_tw");
+    check_hip(hipFree(didx), "hipFree");
+  }
   if (!idx.empty()) {
     int2* didx = nullptr;
     check_hip(hipMalloc(&didx, idx.size() * sizeof(int2)), "hipMalloc(twiddle index)");
@@ -1237,14 +1531,17 @@ void set_len_plan(int n, const std::vector<int>& factors, bool ct_ok, int tile_b
   // the old table is retired, not freed: launches captured earlier (graphs)
   // may still read it; retired tables are freed with the plans
   if (it->second.tw) g_retired_tw.push_back(it->second.tw);
+  if (it->second.twf) g_retired_tw.push_back(it->second.twf);
   it->second = L;  // same map node: references to the entry stay valid
 }
 
 void free_lens() {
   std::lock_guard<std::mutex> lk(g_len_mu);
   (void)hipDeviceSynchronize();
-  for (auto& kv : g_lens)
+  for (auto& kv : g_lens) {
     if (kv.second.tw) (void)hipFree(kv.second.tw);
+    if (kv.second.twf) (void)hipFree(kv.second.twf);
+  }
   for (tw_t* t : g_retired_tw) (void)hipFree(t);
   g_retired_tw.clear();
   g_lens.clear();
@@ -1690,6 +1987,27 @@ static bool x_setup_ct(XArgs& a, const Len1D& L, const CtInfo& c, long long nrow
   return true;
 }
 
+static bool g_x_wave = false;
+void set_x_wave(bool on) { g_x_wave = on; }
+
+// the wave kernels (X rows of 512): four row pairs per workgroup
+static bool launch_x_wave(bool r2c, const void* in, void* out, long long nrows, long long si_row, long long so_row,
+                          int nk, const Len1D& L, bool rev, hipStream_t st) {
+  if (!g_x_wave || !g_use_ct || L.d.n != 512 || !L.twf) return false;
+  // workgroups walk the row pairs at a stride of the grid: at most 8 per CU
+  const long long g = std::min<long long>(((nrows + 1) / 2 + 3) / 4, 256LL * 8);
+  XArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;
+  a.nxo = nk;
+  a.rev = rev ? 1 : 0;
+  const size_t lds = 512 * sizeof(Tw);
+  if (r2c) k_x_r2c_w512<real_t><<<(unsigned)g, 256, lds, st>>>(a, L.twf);
+  else k_x_c2r_w512<real_t><<<(unsigned)g, 256, lds, st>>>(a, L.twf);
+  check_hip(hipGetLastError(), "k_x_w512 launch");
+  return true;
+}
+
 void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_row, long long so_row, int nx,
                   int nxo, hipStream_t st, bool rev) {
   if (nrows <= 0 || nxo <= 0) return;
@@ -1698,6 +2016,7 @@ void launch_x_r2c(const real_t* in, cplx_t* out, long long nrows, long long si_r
     return;
   }
   const Len1D& L = get_len(nx);
+  if (launch_x_wave(true, in, out, nrows, si_row, so_row, nxo, L, rev, st)) return;
   XArgs a;
   std::memset(&a, 0, sizeof(a));
   a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;
@@ -1763,6 +2082,7 @@ void launch_x_c2r(const cplx_t* in, real_t* out, long long nrows, long long si_r
     return;
   }
   const Len1D& L = get_len(nx);
+  if (launch_x_wave(false, in, out, nrows, si_row, so_row, nxi, L, rev, st)) return;
   XArgs a;
   std::memset(&a, 0, sizeof(a));
   a.in = in; a.out = out; a.nrows = nrows; a.si_row = si_row; a.so_row = so_row;

@@ -33,6 +33,7 @@ void check_hip(hipError_t e, const char* what);
 struct Len1D {
   LenDesc d;
   tw_t* tw = nullptr;  // device table: stage twiddles then (if needed) W_n^e
+  tw_t* twf = nullptr; // W_n^e, e < n, for the wave kernels of n = 512
   bool has_generic = false;
   // plan choices (plan-time measurement may change them): specialised
   // kernels allowed; data bytes per tile and bytes of adjacent columns per
@@ -93,6 +94,10 @@ bool launch_x_c2r_tab(const cplx_t* in, real_t* out, long long nrows, int nx, in
 bool launch_xy_plane(bool fwd, const void* in, void* out, int nx, int ny, int nz, hipStream_t st);
 // 0: never; 1: wherever they apply; 2 (default): on small grids only.
 void set_plane_kernels(int mode);
+// X rows of 512 points through the wave kernels (one wavefront per row pair,
+// butterfly exchanges by cross-lane permlane/DPP/bpermute operations) or,
+// by default, the LDS-tile kernels
+void set_x_wave(bool on);
 
 // Column transform along a strided axis of a 3D view [outer][len][inner].
 enum ColKind { COL_C2C = 0, COL_DCT1 = 1, COL_DST1 = 2, COL_COPY = 3 };

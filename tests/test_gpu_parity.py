@@ -921,3 +921,29 @@ def test_cheby_recurrence_in_the_dct_pass(precision, shape, nzc, stride1):
     assert rel(got[0], ref) < tol
     assert rel(got[1], ref) < tol
     assert rel(got[0], got[1]) < tol
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape,nxc", [((512, 6, 4), None), ((512, 3, 3), None), ((512, 5, 2), 300),
+                                       ((512, 64, 8), None), ((512, 160, 129), None)])
+def test_x_wave_kernels_against_oracle(precision, shape, nxc):
+    """X rows of 512 points through the wave kernels (one wavefront per row
+    pair, cross-lane exchanges; switched on) and through the LDS-tile
+    kernels (the default) against the oracle: an odd row count (a last pair
+    with one row), an X cut (fewer stored modes forward, zero modes
+    backward), and more row pairs than the wave kernels' grid (each
+    workgroup walks several)."""
+    nx, ny, nz = shape
+    a = np.random.default_rng(nx + ny + nz).standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz, nxc=nxc)
+    ref_f = o.ftran_r2c(a, "fft")
+    ref_b = o.btran_c2r(ref_f, "tff")[0]
+    lib = p3dfft_amd.load(precision)
+    try:
+        for wave in (1, 0):
+            lib.p3dfft_amd_set_x_wave(wave)
+            with lib_setup(precision, nx, ny, nz, nxc=nxc) as P:
+                assert rel(P.ftran_r2c(a, "fft"), ref_f) < TOL[precision], wave
+                assert rel(P.btran_c2r(ref_f, "tff"), ref_b) < TOL[precision], wave
+    finally:
+        lib.p3dfft_amd_set_x_wave(0)
