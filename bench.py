@@ -306,6 +306,8 @@ def serve_leg(n_req, slots, load_frac=0.7):
         _, mk_seq64, _ = S8.run(order, np.zeros(n_req), sequential=True)
         thr_seq64 = n_req / mk_seq64
         S8.fill()
+        thr_plain64 = n_req / S8.run_plain(order)
+        S8.fill()
         thr_api64 = n_req / S8.run_api(order)
     finally:
         S8.close()
@@ -313,10 +315,14 @@ def serve_leg(n_req, slots, load_frac=0.7):
             "speedup_vs_one_at_a_time": thr_served / thr_seq,
             "one_at_a_time_fp64_solves_per_s": thr_seq64,
             "speedup_vs_one_at_a_time_fp64": thr_served / thr_seq64,
+            "one_at_a_time_plain_fp64_solves_per_s": thr_plain64,
+            "speedup_vs_one_at_a_time_plain_fp64": thr_served / thr_plain64,
             "one_at_a_time_plain_fp64_api_solves_per_s": thr_api64,
             "speedup_vs_one_at_a_time_plain_fp64_api": thr_served / thr_api64,
             "baselines": "one_at_a_time: the same FP32 fused solve, synchronised after each request; "
-                         "fp64: the FP64 fused solve likewise; plain_fp64_api: per request p3dfft_setup, "
+                         "fp64: the FP64 fused solve likewise; plain_fp64: the plain FP64 transform path per "
+                         "request with its plans prepared (forward r2c 'fft', p3dfft_poisson_multiply, backward "
+                         "c2r 'tff': seven passes, no setup), synchronised; plain_fp64_api: per request p3dfft_setup, "
                          "ftran_r2c_dev 'fft', p3dfft_poisson_multiply, btran_c2r_dev 'tff', p3dfft_clean (FP64)",
             "latency_load": "exponential inter-arrival at %.0f%% of served capacity (%.0f req/s)"
                             % (100 * load_frac, load_frac * thr_served),

@@ -246,6 +246,50 @@ class PoissonServer:
         finally:
             L.p3dfft_amd_free(spec)
 
+    def run_plain(self, order):
+        """One request at a time through the plain transform path, plans
+        prepared before the timed loop: the forward r2c 'fft' (X, Y and Z
+        passes) into a spectrum buffer, the per-mode multiply, the backward
+        c2r 'tff' (Z, Y and X passes) back into the field, synchronise; seven
+        HBM passes per solve and no setup or clean per request (the grids'
+        plans are p3dfft_amd_batch_* objects of one grid each).  Returns the
+        makespan (s)."""
+        L = self.lib
+        spec_bytes = max((a // 2 + 1) * b * c for (a, b, c) in self.shapes) * 2 * self.rsize
+        spec = L.p3dfft_amd_malloc(spec_bytes)
+        if not spec:
+            raise P3DFFTError("spectrum allocation failed")
+        plans = {}
+        try:
+            for i in sorted(set(int(j) for j in order)):
+                sh = (ctypes.c_int * 3)(*self.shapes[i])
+                rp = (ctypes.c_void_p * 1)(self.field_ptr(i))
+                sp = (ctypes.c_void_p * 1)(spec)
+                f, b = ctypes.c_void_p(), ctypes.c_void_p()
+                if L.p3dfft_amd_batch_create(1, 1, sh, rp, sp, ctypes.byref(f)):
+                    raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+                plans[i] = (f, None)
+                if L.p3dfft_amd_batch_create(-1, 1, sh, rp, sp, ctypes.byref(b)):
+                    raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+                plans[i] = (f, b)
+            L.p3dfft_amd_synchronize()
+            t0 = time.perf_counter()
+            for i in order:
+                nx, ny, nz = self.shapes[i]
+                f, b = plans[i]
+                if L.p3dfft_amd_batch_launch(f, None) or L.p3dfft_poisson_multiply(spec, nx, ny, nz, None) \
+                        or L.p3dfft_amd_batch_launch(b, None):
+                    raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+                L.p3dfft_amd_synchronize()
+            return time.perf_counter() - t0
+        finally:
+            L.p3dfft_amd_synchronize()
+            for f, b in plans.values():
+                for h in (f, b):
+                    if h is not None and h.value:
+                        L.p3dfft_amd_batch_destroy(h)
+            L.p3dfft_amd_free(spec)
+
     def batch(self, idx):
         """A PoissonBatch (grouped launches) of the solves of requests `idx`."""
         return PoissonBatch(self, idx)

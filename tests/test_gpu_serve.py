@@ -362,3 +362,23 @@ def test_grouped_batch_on_the_served_size_mix():
         if B is not None:
             B.close()
         S.close()
+
+
+def test_plain_path_baseline_same_solves():
+    """The bench's plain-path baseline (PoissonServer.run_plain: forward r2c,
+    the per-mode multiply and backward c2r per request, through one-grid
+    plans) computes the same solves as the fused entry point: FP64 within
+    1e-12 relative L2."""
+    shapes = serve.make_requests(12, seed=13, sizes=(32, 48, 64, 96, 128, 160))
+    S = serve.PoissonServer(shapes, precision=8, slots=1)
+    try:
+        order = list(range(len(shapes)))
+        S.fill()
+        S.run(order, np.zeros(len(shapes)), sequential=True)
+        fused = [S.download(i).astype(np.float64) for i in order]
+        S.fill()
+        assert S.run_plain(order) > 0
+        for i in order:
+            assert rel(S.download(i), fused[i]) < 1e-12, (i, shapes[i])
+    finally:
+        S.close()
