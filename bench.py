@@ -22,8 +22,8 @@ Prints ONE JSON line.  Besides metric, value, unit, steps and timing:
                  one-at-a-time baselines
   batched        a fixed batch of small solves: eager streams, one HIP
                  graph, grouped launches, one at a time
-  cpu_baseline   the FP64 CPU oracle (a restatement of the reference's CPU
-                 algorithm) on a bounded sample, host cores of this machine
+  cpu_baseline   the reference's distributed CPU path restated by the FP64 C
+                 oracle, one rank per host core on the drivers' P1 x P2 grid
 """
 import argparse
 import json
@@ -167,32 +167,34 @@ def copy_peak(lib, nbytes=2 << 30, reps=10):
 
 
 def cpu_baseline(n, f, threads, loops=5):
-    """FP64 CPU oracle (restatement of the reference CPU algorithm, OpenMP over
-    1D lines) timed as the reference's driver loop times its transforms
-    (sample/FORTRAN/driver_sine.F90:192-232: warm-up, then n x (ftran, btran)
-    with the normalisation outside the timed calls): one warm-up pair, then
-    `loops` timed pairs on the same field; the median pair is the value."""
+    """The reference's distributed CPU path restated by the C oracle, FP64:
+    `threads` ranks on the drivers' P1 x P2 grid (MPI_Dims_create, dims(1) <=
+    dims(2); sample/FORTRAN/driver_rand.F90:115-137), one rank per host core
+    (each rank single-threaded), the four pencil exchanges as shared-memory
+    copies of the fcomm1/fcomm2/bcomm1/bcomm2 buffers; timed as the
+    reference's driver loop times its transforms
+    (sample/FORTRAN/driver_sine.F90:192-232: warm-up, then n x (ftran, btran),
+    barrier to barrier): one warm-up pair, then `loops` timed pairs on the
+    same field; the median pair is the value.  The OpenMP-over-lines 1 x 1
+    oracle is timed beside it (one pair) for comparison."""
     from oracle import oracle as O
+    p1, p2 = O.dims_create(threads)
+    _, _, tf, tb = O.ranks_fwd_bwd(f, p1, p2, reps=loops + 1, spectrum=False, back=False)
+    tf, tb = tf[1:], tb[1:]
+    pairs = [float(a + b) for a, b in zip(tf, tb)]
+    med = float(np.median(pairs))
     O.set_threads(threads)
     p = O.Plan(n, n, n)
-    s = p.ftran_r2c(f, "fft")
-    p.btran_c2r(s, "tff", keep_input=False)
-    tf, tb = [], []
-    for _ in range(loops):
-        t0 = time.perf_counter()
-        s = p.ftran_r2c(f, "fft")
-        t1 = time.perf_counter()
-        p.btran_c2r(s, "tff", keep_input=False)
-        tf.append(t1 - t0)
-        tb.append(time.perf_counter() - t1)
-    pairs = [a + b for a, b in zip(tf, tb)]
-    med = float(np.median(pairs))
-    return {"value": 1.0 / med, "unit": "fwd+bwd/s", "cores": O.get_threads(), "kind": "port",
-            "sample": "the reference driver's timing loop on the same %d^3 field in FP64 by the C oracle: "
-                      "one warm-up pair, then %d x (ftran 'fft', btran 'tff'); median pair %.2f s "
-                      "(ftran %.2f s, btran %.2f s median)"
-                      % (n, loops, med, float(np.median(tf)), float(np.median(tb))),
-            "pairs_s": pairs}
+    t0 = time.perf_counter()
+    p.btran_c2r(p.ftran_r2c(f, "fft"), "tff", keep_input=False)
+    t_lines = time.perf_counter() - t0
+    return {"value": 1.0 / med, "unit": "fwd+bwd/s", "cores": threads, "kind": "port",
+            "sample": "the reference's CPU path on a %d x %d grid of ranks (one per host core, threads of one "
+                      "process, exchanges as shared-memory copies of the pack/unpack buffers) on the same "
+                      "%d^3 field in FP64 by the C oracle, timed like the reference driver: one warm-up pair, "
+                      "then %d x (ftran 'fft', btran 'tff'); median pair %.2f s (ftran %.2f s, btran %.2f s "
+                      "median)" % (p1, p2, n, loops, med, float(np.median(tf)), float(np.median(tb))),
+            "grid": [p1, p2], "pairs_s": pairs, "one_rank_openmp_lines_pair_s": t_lines}
 
 
 def fp64_poisson(f):
@@ -454,21 +456,18 @@ def cpu_baseline_cheby(a, Lz, got, back, threads, loops=3):
 
 
 def cpu_baseline_small(n, threads, loops=20):
-    """The CPU-runnable 64^3 FP64 case by the CPU oracle, timed like the
-    reference driver (one warm-up pair, then `loops` pairs; the median)."""
+    """The CPU-runnable 64^3 FP64 case by the oracle's distributed CPU path
+    (`threads` ranks on the drivers' P1 x P2 grid, as cpu_baseline), timed
+    like the reference driver: one warm-up pair, then `loops` pairs; the
+    median."""
     from oracle import oracle as O
-    O.set_threads(threads)
+    p1, p2 = O.dims_create(threads)
     f = np.random.default_rng(64).random((n, n, n))
-    p = O.Plan(n, n, n)
-    p.btran_c2r(p.ftran_r2c(f, "fft"), "tff", keep_input=False)
-    tt = []
-    for _ in range(loops):
-        t0 = time.perf_counter()
-        p.btran_c2r(p.ftran_r2c(f, "fft"), "tff", keep_input=False)
-        tt.append(time.perf_counter() - t0)
-    med = float(np.median(tt))
-    return {"value": 1.0 / med, "unit": "fwd+bwd/s", "cores": O.get_threads(), "kind": "port",
-            "sample": "64^3 U[0,1) FP64, one warm-up pair then %d pairs; median %.4f s" % (loops, med)}
+    _, _, tf, tb = O.ranks_fwd_bwd(f, p1, p2, reps=loops + 1, spectrum=False, back=False)
+    med = float(np.median(tf[1:] + tb[1:]))
+    return {"value": 1.0 / med, "unit": "fwd+bwd/s", "cores": threads, "kind": "port",
+            "sample": "64^3 U[0,1) FP64 on a %d x %d grid of ranks, one warm-up pair then %d pairs; median %.5f s"
+                      % (p1, p2, loops, med)}
 
 
 def small_leg(n=64, reps=200):

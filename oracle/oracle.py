@@ -69,10 +69,42 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [ctypes.POINTER(OrcPlan), _c_dbl_p, _c_dbl_p]
         L.orc_share_counts.argtypes = [ctypes.POINTER(OrcPlan)] + [_c_ll_p] * 8
+        L.orc_ranks_fwd_bwd.argtypes = [ctypes.c_int] * 5 + [_c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
+                                                             _c_dbl_p, _c_dbl_p]
+        L.orc_ranks_fwd_bwd.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def dims_create(nprocs):
+    """The sample drivers' processor grid: MPI_Dims_create(nprocs, 2) (the
+    most balanced factorisation, larger factor first), then dims(1) <=
+    dims(2) (sample/FORTRAN/driver_rand.F90:115-137)."""
+    a = int(nprocs ** 0.5)
+    while nprocs % a:
+        a -= 1
+    return (a, nprocs // a)
+
+
+def ranks_fwd_bwd(a, p1, p2, reps=1, spectrum=True, back=True):
+    """The distributed CPU path ('fft' then 'tff', no cuts) on a p1 x p2
+    grid with the ranks as threads.  a: real field (nz, ny, nx).  Returns
+    (spectrum (nz, ny, nx/2+1) or None, back (nz, ny, nx) or None, forward
+    times, backward times) in seconds."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    nz, ny, nx = a.shape
+    spec = np.zeros((nz, ny, nx // 2 + 1), dtype=np.complex128) if spectrum else None
+    out = np.zeros_like(a) if back else None
+    tf = np.zeros(reps)
+    tb = np.zeros(reps)
+    rc = lib().orc_ranks_fwd_bwd(nx, ny, nz, p1, p2, _dp(a),
+                                 _dp(spec.view(np.float64)) if spectrum else None,
+                                 _dp(out) if back else None, reps, _dp(tf), _dp(tb))
+    if rc:
+        raise ValueError("oracle ranks_fwd_bwd failed (%d)" % rc)
+    return spec, out, tf, tb
 
 
 def _dp(a):

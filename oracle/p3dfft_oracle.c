@@ -905,3 +905,209 @@ void orc_bcomm2_unpack(const orc_plan *p, const double *recv, double *dest) {
       }
   splits_free(&s);
 }
+
+/* ------------------------------------------------------------------ */
+/* The reference's distributed CPU path on a p1 x p2 grid, with the ranks
+ * as OpenMP threads of one process (each rank single-threaded, as one MPI
+ * rank is): per rank X r2c of its X pencil (ftran.F90:525-551), fcomm1
+ * pack, the row exchange (the MPI_Alltoallv of fcomm1.F90:255-262 as
+ * copies between the ranks' buffers), fcomm1 unpack, Y c2c, fcomm2 pack,
+ * the column exchange (fcomm2.F90:313), Z c2c on the receive buffer (the Z
+ * pencil); backward: Z c2c, the bcomm1 exchange (bcomm1.F90:295-296) and
+ * unpack, Y c2c, bcomm2 pack, exchange and unpack, X c2r
+ * (btran.F90:396-679).  Ops 'fft' / 'tff', no cuts.  in: global real
+ * (nx, ny, nz); spec (optional): the global spectrum (nxhp, ny, nz)
+ * assembled from the Z pencils after the last forward; out: global real
+ * after the last backward.  t_fwd / t_bwd (reps each): wall seconds of each
+ * forward / backward, from barrier to barrier.  Returns 0, or -1 on invalid
+ * input.                                                               */
+/* ------------------------------------------------------------------ */
+static void ranks_line(const fplan *fp, double *base, long long stride, int sign, double *y, double *tmp) {
+  int j;
+  fft_line(fp, base, stride, y, sign, tmp);
+  for (j = 0; j < fp->n; j++) { base[2 * j * stride] = y[2 * j]; base[2 * j * stride + 1] = y[2 * j + 1]; }
+}
+
+int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, double *spec, double *out,
+                      int reps, double *t_fwd, double *t_bwd) {
+  int P = p1 * p2, ok = 1;
+  orc_plan *pl;
+  double **xr, **b2, **s1, **r1, **yp, **s2, **zp;
+  const fplan *fx, *fy, *fz;
+  int mf;
+  if (nx <= 0 || ny <= 0 || nz <= 0 || p1 <= 0 || p2 <= 0 || reps <= 0) return -1;
+  pl = (orc_plan *)calloc(P, sizeof(orc_plan));
+  {
+    int dims[2] = {p1, p2}, r;
+    for (r = 0; r < P; r++)
+      if (orc_setup(&pl[r], dims, r % p1, r / p1, nx, ny, nz, nx, ny, nz, 1, 0)) ok = 0;
+    for (r = 0; r < P; r++)
+      if (pl[r].iisize <= 0 || pl[r].jisize <= 0 || pl[r].jjsize <= 0 || pl[r].kjsize <= 0) ok = 0;
+  }
+  if (!ok) { free(pl); return -1; }
+  fx = get_plan(nx); fy = get_plan(ny); fz = get_plan(nz);
+  mf = max_factor(fx);
+  if (max_factor(fy) > mf) mf = max_factor(fy);
+  if (max_factor(fz) > mf) mf = max_factor(fz);
+  xr = (double **)calloc(P, sizeof(double *)); b2 = (double **)calloc(P, sizeof(double *));
+  s1 = (double **)calloc(P, sizeof(double *)); r1 = (double **)calloc(P, sizeof(double *));
+  yp = (double **)calloc(P, sizeof(double *)); s2 = (double **)calloc(P, sizeof(double *));
+  zp = (double **)calloc(P, sizeof(double *));
+#ifdef _OPENMP
+#pragma omp parallel num_threads(P)
+#endif
+  {
+#ifdef _OPENMP
+    const int r = omp_get_thread_num();
+#else
+    const int r = 0;
+#endif
+    const orc_plan *p = &pl[r];
+    const int ip = p->ipid, jp = p->jpid;
+    const long long nrows = (long long)p->jisize * p->kjsize;
+    const long long nb2 = (long long)p->nxhp * nrows;
+    const long long nyp = (long long)p->iisize * ny * p->kjsize;
+    const long long nzp = (long long)p->iisize * p->jjsize * nz;
+    long long *fs_st = malloc(sizeof(long long) * p1), *fs_ct = malloc(sizeof(long long) * p1);
+    long long *fr_st = malloc(sizeof(long long) * p1), *fr_ct = malloc(sizeof(long long) * p1);
+    long long *ks_st = malloc(sizeof(long long) * p2), *ks_ct = malloc(sizeof(long long) * p2);
+    long long *kr_st = malloc(sizeof(long long) * p2), *kr_ct = malloc(sizeof(long long) * p2);
+    double *c = malloc(sizeof(double) * 2 * (nx + ny + nz));
+    double *y = malloc(sizeof(double) * 2 * (nx + ny + nz));
+    double *tmp = malloc(sizeof(double) * 2 * (mf + 2));
+    long long l, q;
+    int it, i, j;
+    orc_share_counts(p, fs_st, fs_ct, fr_st, fr_ct, ks_st, ks_ct, kr_st, kr_ct);
+    xr[r] = malloc(sizeof(double) * nx * nrows);
+    b2[r] = malloc(sizeof(double) * 2 * nb2);
+    s1[r] = malloc(sizeof(double) * 2 * (long long)p->nxhpc * nrows);
+    r1[r] = malloc(sizeof(double) * 2 * nyp);
+    yp[r] = malloc(sizeof(double) * 2 * nyp);
+    s2[r] = malloc(sizeof(double) * 2 * nyp);
+    zp[r] = malloc(sizeof(double) * 2 * nzp);
+    /* this rank's X pencil of the global field */
+    for (l = 0; l < nrows; l++) {
+      const long long yy = p->jistart - 1 + l % p->jisize, zz = p->kjstart - 1 + l / p->jisize;
+      memcpy(xr[r] + l * nx, in + (yy + zz * (long long)ny) * nx, sizeof(double) * nx);
+    }
+#ifdef _OPENMP
+#pragma omp barrier
+#endif
+    for (it = 0; it < reps; it++) {
+      double t0 = 0;
+#ifdef _OPENMP
+#pragma omp barrier
+      if (r == 0) t0 = omp_get_wtime();
+#endif
+      /* X r2c */
+      for (l = 0; l < nrows; l++) {
+        for (j = 0; j < nx; j++) { c[2 * j] = xr[r][l * nx + j]; c[2 * j + 1] = 0.0; }
+        fft_line(fx, c, 1, y, -1, tmp);
+        memcpy(b2[r] + 2 * l * p->nxhp, y, sizeof(double) * 2 * p->nxhp);
+      }
+      orc_fcomm1_pack(p, b2[r], s1[r]);
+#ifdef _OPENMP
+#pragma omp barrier
+#endif
+      /* row exchange: the block from row peer i (same jpid) */
+      for (i = 0; i < p1; i++) {
+        const orc_plan *pp = &pl[jp * p1 + i];
+        const long long off = (long long)(p->iistart - 1) * pp->jisize * pp->kjsize;
+        memcpy(r1[r] + 2 * fr_st[i], s1[jp * p1 + i] + 2 * off, sizeof(double) * 2 * fr_ct[i]);
+      }
+      orc_fcomm1_unpack(p, r1[r], yp[r]);
+      /* Y c2c */
+      for (q = 0; q < (long long)p->iisize * p->kjsize; q++) {
+        const long long xx = q % p->iisize, zz = q / p->iisize;
+        ranks_line(fy, yp[r] + 2 * (xx + zz * (long long)p->iisize * ny), p->iisize, -1, y, tmp);
+      }
+      orc_fcomm2_pack(p, yp[r], s2[r]);
+#ifdef _OPENMP
+#pragma omp barrier
+#endif
+      /* column exchange: the block from column peer j (same ipid) lands at
+       * its z range of the Z pencil */
+      for (j = 0; j < p2; j++) {
+        const orc_plan *pp = &pl[j * p1 + ip];
+        const long long off = (long long)(p->jjstart - 1) * pp->iisize * pp->kjsize;
+        memcpy(zp[r] + 2 * kr_st[j], s2[j * p1 + ip] + 2 * off, sizeof(double) * 2 * kr_ct[j]);
+      }
+      /* Z c2c */
+      for (q = 0; q < (long long)p->iisize * p->jjsize; q++)
+        ranks_line(fz, zp[r] + 2 * q, (long long)p->iisize * p->jjsize, -1, y, tmp);
+#ifdef _OPENMP
+#pragma omp barrier
+      if (r == 0) { t_fwd[it] = omp_get_wtime() - t0; t0 = omp_get_wtime(); }
+#endif
+      if (spec && it == reps - 1) {
+        for (q = 0; q < (long long)p->jjsize * nz; q++) {
+          const long long yy = p->jjstart - 1 + q % p->jjsize, zz = q / p->jjsize;
+          memcpy(spec + 2 * (p->iistart - 1 + p->nxhp * (yy + (long long)ny * zz)), zp[r] + 2 * q * p->iisize,
+                 sizeof(double) * 2 * p->iisize);
+        }
+#ifdef _OPENMP
+#pragma omp barrier
+        if (r == 0) t0 = omp_get_wtime();
+#endif
+      }
+      /* backward: Z c2c */
+      for (q = 0; q < (long long)p->iisize * p->jjsize; q++)
+        ranks_line(fz, zp[r] + 2 * q, (long long)p->iisize * p->jjsize, +1, y, tmp);
+#ifdef _OPENMP
+#pragma omp barrier
+#endif
+      /* bcomm1: from column peer j its slice z in this rank's z range of
+       * its Z pencil, into this rank's receive buffer at the fcomm2 send
+       * displacement of j */
+      for (j = 0; j < p2; j++) {
+        const orc_plan *pp = &pl[j * p1 + ip];
+        const long long off = (long long)(p->kjstart - 1) * pp->iisize * pp->jjsize;
+        memcpy(s2[r] + 2 * ks_st[j], zp[j * p1 + ip] + 2 * off, sizeof(double) * 2 * ks_ct[j]);
+      }
+      orc_bcomm1_unpack(p, s2[r], yp[r]);
+      /* Y c2c */
+      for (q = 0; q < (long long)p->iisize * p->kjsize; q++) {
+        const long long xx = q % p->iisize, zz = q / p->iisize;
+        ranks_line(fy, yp[r] + 2 * (xx + zz * (long long)p->iisize * ny), p->iisize, +1, y, tmp);
+      }
+      orc_bcomm2_pack(p, yp[r], r1[r]);
+#ifdef _OPENMP
+#pragma omp barrier
+#endif
+      /* bcomm2: from row peer i its block for this rank, into the receive
+       * buffer at the fcomm1 send displacement of i */
+      for (i = 0; i < p1; i++) {
+        const orc_plan *pp = &pl[jp * p1 + i];
+        const long long off = (long long)(p->jistart - 1) * pp->iisize * pp->kjsize;
+        memcpy(s1[r] + 2 * fs_st[i], r1[jp * p1 + i] + 2 * off, sizeof(double) * 2 * fs_ct[i]);
+      }
+      orc_bcomm2_unpack(p, s1[r], b2[r]);
+      /* X c2r */
+      for (l = 0; l < nrows; l++) c2r_line(fx, b2[r] + 2 * l * p->nxhp, 1, xr[r] + l * nx, 1, c, y, tmp);
+#ifdef _OPENMP
+#pragma omp barrier
+      if (r == 0) t_bwd[it] = omp_get_wtime() - t0;
+#endif
+      /* the next forward transforms the input again */
+      if (it < reps - 1 || out) {
+        if (out && it == reps - 1)
+          for (l = 0; l < nrows; l++) {
+            const long long yy = p->jistart - 1 + l % p->jisize, zz = p->kjstart - 1 + l / p->jisize;
+            memcpy(out + (yy + zz * (long long)ny) * nx, xr[r] + l * nx, sizeof(double) * nx);
+          }
+        for (l = 0; l < nrows; l++) {
+          const long long yy = p->jistart - 1 + l % p->jisize, zz = p->kjstart - 1 + l / p->jisize;
+          memcpy(xr[r] + l * nx, in + (yy + zz * (long long)ny) * nx, sizeof(double) * nx);
+        }
+      }
+    }
+#ifdef _OPENMP
+#pragma omp barrier
+#endif
+    free(xr[r]); free(b2[r]); free(s1[r]); free(r1[r]); free(yp[r]); free(s2[r]); free(zp[r]);
+    free(fs_st); free(fs_ct); free(fr_st); free(fr_ct); free(ks_st); free(ks_ct); free(kr_st); free(kr_ct);
+    free(c); free(y); free(tmp);
+  }
+  free(xr); free(b2); free(s1); free(r1); free(yp); free(s2); free(zp); free(pl);
+  return 0;
+}

@@ -24,6 +24,9 @@
  *                         (fft_exec.F90:646-701, 866-921).
  *   share-mode maps       build/fcomm1.F90:209-326, fcomm2.F90:250-386,
  *                         bcomm1.F90:250-380, bcomm2.F90:213-318
+ *   orc_ranks_fwd_bwd     ftran.F90:489-780 and btran.F90:396-679 on a
+ *                         p1 x p2 grid, the ranks as threads, the
+ *                         MPI_Alltoallv exchanges as copies between them
  */
 #ifndef P3DFFT_ORACLE_H
 #define P3DFFT_ORACLE_H
@@ -99,6 +102,14 @@ void orc_share_counts(const orc_plan *p, long long *fsnd_st, long long *fsnd_ct,
                       long long *frcv_st, long long *frcv_ct,
                       long long *ksnd_st, long long *ksnd_ct,
                       long long *krcv_st, long long *krcv_ct);
+
+/* ---- the distributed CPU path, p1 x p2 ranks as OpenMP threads ---- */
+/* ops 'fft' / 'tff' without cuts; in: global real (nx,ny,nz); spec
+ * (optional): global spectrum (nxhp,ny,nz) after the last forward; out
+ * (optional): global real after the last backward; t_fwd / t_bwd: reps wall
+ * times (s).  Returns 0 or -1. */
+int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, double *spec, double *out,
+                      int reps, double *t_fwd, double *t_bwd);
 
 /* number of OpenMP threads the oracle uses (0 = leave default) */
 void orc_set_threads(int n);

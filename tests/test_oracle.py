@@ -490,3 +490,27 @@ def test_share_pack_order_against_hand_worked_buffers():
     assert np.array_equal(o.fcomm1_pack(src).real, np.array(h["fcomm1_send"], dtype=np.float64))
     ypen = np.arange(3 * 6 * 3, dtype=np.float64) + 0j
     assert np.array_equal(o.bcomm2_pack(ypen).real, np.array(h["bcomm2_send"], dtype=np.float64))
+
+
+@pytest.mark.parametrize("grid", [(2, 3), (3, 2), (1, 4), (4, 1), (2, 2)])
+def test_ranks_path_reproduces_single_rank(grid):
+    """The distributed CPU path (ranks as threads, exchanges as copies of the
+    pack/unpack maps' buffers) on the reference's uneven 14 x 26 x 38 grid:
+    the assembled spectrum equals the single-rank oracle's and numpy's
+    rfftn, the round trip returns N times the field, for every grid shape
+    (each rank's X, Y and Z pencils and all four exchanges)."""
+    nx, ny, nz = 14, 26, 38
+    a = np.random.default_rng(sum(grid)).standard_normal((nz, ny, nx))
+    spec, back, tf, tb = O.ranks_fwd_bwd(a, *grid, reps=2)
+    ref = O.Plan(nx, ny, nz).ftran_r2c(a)
+    assert rel(spec, ref) < 1e-13
+    assert rel(spec, np.fft.rfftn(a)) < 1e-13
+    assert rel(back, a * (nx * ny * nz)) < 1e-13
+    assert np.all(tf > 0) and np.all(tb > 0)
+
+
+def test_dims_create_like_the_drivers():
+    assert O.dims_create(16) == (4, 4)
+    assert O.dims_create(12) == (3, 4)
+    assert O.dims_create(8) == (2, 4)
+    assert O.dims_create(7) == (1, 7)
