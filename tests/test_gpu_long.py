@@ -95,3 +95,71 @@ def test_long_pruned_and_stride1(precision):
         with lib_setup(precision, nx, ny, nz, **kw) as P:
             assert rel(P.ftran_r2c(f, "fft"), ref) < TOL[precision], cs
             assert rel(P.btran_c2r(ref, "tff"), o.btran_c2r(ref, "tff")[0]) < TOL[precision], cs
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_long_many_and_1d(precision):
+    """The multivariable entry points with a long Z length (nv = 2, padded
+    dim_in / dim_out) against the oracle variable by variable, and the X
+    r2c stage alone (ftran_r2c_1d) of a long X length against numpy."""
+    import p3dfft_amd
+    nx, ny, nz = 4, 3, 16384
+    o = oplan(nx, ny, nz)
+    nr, nc = nx * ny * nz, int(np.prod(o.spec_shape()))
+    nv, pad = 2, 5
+    a = np.random.default_rng(7).standard_normal(nv * (nr + pad))
+    with lib_setup(precision, nx, ny, nz) as P:
+        s = P.ftran_r2c_many(a, nr + pad, nc + pad, nv, "fft")
+        for j in range(nv):
+            ref = o.ftran_r2c(a[j * (nr + pad):j * (nr + pad) + nr], "fft").reshape(-1)
+            assert rel(s[j * (nc + pad):j * (nc + pad) + nc], ref) < TOL[precision], j
+    nx, ny, nz = 9000, 2, 3
+    f = np.random.default_rng(8).standard_normal((nz, ny, nx))
+    P = p3dfft_amd.P3DFFT(precision)
+    P.setup((1, 1), nx, ny, nz)
+    try:
+        got = P.ftran_r2c_1d(f).reshape(nz, ny, nx // 2 + 1)
+    finally:
+        P.clean()
+    assert rel(got, np.fft.rfft(f, axis=2)) < TOL[precision]
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_long_capture_after_eager_run(precision):
+    """A transform with a long Z length captured into a graph after it ran
+    once eagerly (plans and scratch exist) replays bit for bit, three times
+    with eager transforms between the replays."""
+    import ctypes
+    nx, ny, nz = 4, 3, 16384
+    rng = np.random.default_rng(60)
+    f = rng.standard_normal((nz, ny, nx))
+    g2 = rng.standard_normal((nz, ny, nx))
+    with lib_setup(precision, nx, ny, nz) as P:
+        L = P.lib
+        st = L.p3dfft_amd_stream_create()
+        fin, spec, out = P.alloc(P.real_bytes()), P.alloc(P.spec_bytes()), P.alloc(P.real_bytes())
+        other, ospec = P.alloc(P.real_bytes()), P.alloc(P.spec_bytes())
+        graph = ctypes.c_void_p()
+        try:
+            fin.upload(f.astype(P.rdtype))
+            P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft", st)
+            P.btran_c2r_dev(spec.ptr, out.ptr, "tff", st)
+            L.p3dfft_amd_stream_synchronize(st)
+            b0 = out.download(P.real_shape(), P.rdtype)
+            assert L.p3dfft_amd_capture_begin(st) == 0
+            P.ftran_r2c_dev(fin.ptr, spec.ptr, "fft", st)
+            P.btran_c2r_dev(spec.ptr, out.ptr, "tff", st)
+            assert L.p3dfft_amd_capture_end(st, ctypes.byref(graph)) == 0
+            for rep in range(3):
+                other.upload(g2.astype(P.rdtype))
+                P.ftran_r2c_dev(other.ptr, ospec.ptr, "fft", st)
+                out.upload(np.zeros((nz, ny, nx), P.rdtype))
+                assert L.p3dfft_amd_graph_launch(graph, st) == 0
+                L.p3dfft_amd_stream_synchronize(st)
+                assert np.array_equal(out.download(P.real_shape(), P.rdtype), b0), rep
+        finally:
+            if graph.value:
+                L.p3dfft_amd_graph_destroy(graph)
+            for b in (fin, spec, out, other, ospec):
+                b.free()
+            L.p3dfft_amd_stream_destroy(st)
