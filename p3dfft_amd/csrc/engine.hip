@@ -167,6 +167,7 @@ __device__ void run_fft(cpx<T>* lds, const LenDesc& L, int B, FDiv dB, const typ
       case 8: stage_r<T, 8, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
       case 11: stage_r<T, 11, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
       case 13: stage_r<T, 13, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
+      case 16: stage_r<T, 16, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
       default: stage_g<T, FWD, COL, NT, EPT>(lds, L, s, B, dB, tw); break;
     }
   }
@@ -1423,6 +1424,8 @@ static FDiv make_fdiv(int d) {
 // stage radices: radix 8 as far as possible (two 4s instead of an 8 and a 2),
 // then 3, 5, 7, 11, 13, then any remaining prime (generic stage)
 static std::vector<int> factorize(int n) {
+  // 2048: three stages (two LDS exchanges) with radix 16
+  if (n == 2048) return {16, 16, 8};
   std::vector<int> f;
   int m = n, a = 0;
   while (m % 2 == 0) { m /= 2; a++; }
@@ -1440,7 +1443,7 @@ static std::vector<int> factorize(int n) {
 }
 
 static bool is_templated(int r) {
-  return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 11 || r == 13;
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 11 || r == 13 || r == 16;
 }
 
 static std::mutex g_len_mu;
@@ -1850,7 +1853,7 @@ static const CtInfo* find_ct(const Len1D& L) {
       {96, CtLen<8, 4, 3>::info()},    {128, CtLen<8, 4, 4>::info()},   {160, CtLen<8, 4, 5>::info()},
       {192, CtLen<8, 8, 3>::info()},   {256, CtLen<8, 8, 4>::info()},   {320, CtLen<8, 8, 5>::info()},
       {384, CtLen<8, 4, 4, 3>::info()}, {512, CtLen<8, 8, 8>::info()},  {1024, CtLen<8, 8, 4, 4>::info()},
-      {2048, CtLen<8, 8, 8, 4>::info()}};
+      {2048, CtLen<16, 16, 8>::info()}};
   auto it = tab.find(L.d.n);
   if (it == tab.end()) return nullptr;
   const CtInfo& c = it->second;

@@ -101,6 +101,52 @@ __device__ __forceinline__ void dft8(cpx<T>* v) {
   v[7] = sub(e[3], o3);
 }
 
+// 16 points as 4 x 4: DFT4 over the stride-4 subsequences n1 + 4 n2, the
+// twiddles W16^(n1 k1), DFT4 over n1; output in natural order
+template <bool FWD, class T>
+__device__ __forceinline__ void dft16(cpx<T>* v) {
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173;
+  const T h = (T)0.70710678118654752440;
+  cpx<T> y[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; n1++) {
+    cpx<T> t[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+    dft4<FWD>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) y[n1][k1] = t[k1];
+  }
+  // W16^m (forward exp(-2 pi i m / 16), backward its conjugate), m = n1 k1
+  auto w = [&](cpx<T> a, int m) -> cpx<T> {
+    T c, sn;
+    switch (m) {
+      case 1: c = c1; sn = s1; break;
+      case 2: c = h; sn = h; break;
+      case 3: c = s1; sn = c1; break;
+      case 4: c = 0; sn = 1; break;
+      case 6: c = -h; sn = h; break;
+      default: c = -c1; sn = -s1; break;  // m = 9
+    }
+    const T sy = FWD ? -sn : sn;
+    return mk<T>(a.x * c - a.y * sy, a.x * sy + a.y * c);
+  };
+  y[1][1] = w(y[1][1], 1);
+  y[1][2] = w(y[1][2], 2);
+  y[1][3] = w(y[1][3], 3);
+  y[2][1] = w(y[2][1], 2);
+  y[2][2] = rot<FWD>(y[2][2]);
+  y[2][3] = w(y[2][3], 6);
+  y[3][1] = w(y[3][1], 3);
+  y[3][2] = w(y[3][2], 6);
+  y[3][3] = w(y[3][3], 9);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) {
+    cpx<T> t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4<FWD>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
 template <bool FWD, class T>
 __device__ __forceinline__ void dft3(cpx<T>* v) {
   const T c3 = (T)0.86602540378443864676;
@@ -188,6 +234,7 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
   else if constexpr (R == 4) dft4<FWD>(v);
   else if constexpr (R == 5) dft5<FWD>(v);
   else if constexpr (R == 8) dft8<FWD>(v);
+  else if constexpr (R == 16) dft16<FWD>(v);
   else dftp<R, FWD>(v);
 }
 
