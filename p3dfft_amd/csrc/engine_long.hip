@@ -305,6 +305,7 @@ long long smooth235(long long n) {
 }
 
 void* long_scratch(size_t bytes, hipStream_t st) {
+  std::lock_guard<std::recursive_mutex> lk(g_long_mu);
   if (bytes <= g_ls_bytes) return g_ls;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   check_hip(hipStreamIsCapturing(st, &cs), "hipStreamIsCapturing");
@@ -397,6 +398,7 @@ LongPlan build_long(int m) {
   L.M = (int)M;
   // the convolution's own four-step plan (M is 5-smooth)
   const LongPlan& LM = get_long((int)M);
+  if (!LM.N1) throw Error{"transform length " + std::to_string(m) + " is not supported", -1};
   a = LM.N1;
   b = LM.N2;
   // the filter's transform, once per length (FFT of length M in the build's
