@@ -278,6 +278,12 @@ void p3dfft_amd_set_plane_kernels(int mode);
  * no workgroup barrier), 0 (default) through the LDS-tile kernels, which
  * measured faster.  No reference counterpart; results agree to rounding. */
 void p3dfft_amd_set_x_wave(int on);
+/* DCT-I Z passes ('c' ops) without the fused Chebyshev recurrence: 1 (default)
+ * through the direct-load column kernel, which reads the even extension's
+ * mirrored rows from the input; 0 through the kernel that builds the
+ * extension in LDS first.  No reference counterpart; results agree to
+ * rounding. */
+void p3dfft_amd_set_dct_direct(int on);
 /* 1 (default): a forward without Y/Z cuts whose output does not overlap its
  * input runs X r2c straight into the output and the Y pass in place there
  * (the Z pass then reads lines the Y pass left in the last-level cache);

@@ -303,6 +303,9 @@ struct ColArgs {
   // of its per-wave sums behind the tile
   int cheby, cheby_off;
   double cheby_scale, cheby_L;
+  // > 0: the direct-load kernel reads a DCT-I even extension of this many
+  // input rows (row e <- row e, e < mirror; row 2(mirror-1) - e above)
+  int mirror;
 };
 
 template <class T, bool FWD, int NT, int EPT>
@@ -940,6 +943,7 @@ __device__ __forceinline__ ColIO col_io(const ColArgs& a, long long c0, int nb) 
   io.nb = nb;
   io.ld_lo = a.ld_lo; io.ld_hi = a.ld_hi; io.ld_shift = a.ld_shift;
   io.nst = a.nst; io.st_lo = a.st_lo; io.st_shift = a.st_shift;
+  io.mirror = a.mirror;
   io.lane_o = o; io.lane_i = i;
   io.ltab = a.ltab; io.stab = a.stab;
   if (a.ltab) {  // table-addressed rows index the whole arrays
@@ -1934,6 +1938,8 @@ static bool g_use_ct = true;
 static bool g_alt_order = true;
 static int g_flat_order = 2;  // 0 XCD-aware, 1 dispatch order, 2 by shape
 static bool g_wide_offsets = false;
+static bool g_dct_direct = true;
+void set_dct_direct(bool on) { g_dct_direct = on; }
 void set_wide_offsets(bool on) { g_wide_offsets = on; }
 void set_flat_order(int mode) { g_flat_order = mode < 0 || mode > 2 ? 2 : mode; }
 int flat_order() { return g_flat_order; }
@@ -2320,6 +2326,16 @@ bool launch_col(const ColSpec& c, hipStream_t st) {
     if (c.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast && (fits || !pr)) {
       ct->colio(fwd, pr, !fits, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
       check_hip(hipGetLastError(), "k_colio_ct launch");
+      return false;
+    }
+    // DCT-I without the fused Chebyshev recurrence: the direct-load kernel
+    // reads the even extension's mirrored rows from the input itself (no LDS
+    // load pass); the store map keeps the first nst rows
+    const bool ld_all = c.nld == c.n && (c.ld_lo >= c.n || (c.ld_hi <= c.ld_lo && c.ld_shift == 0));
+    if (c.kind == COL_DCT1 && !c.cheby && g_dct_direct && ld_all && !a.ld_lenfast && !a.st_lenfast && fits) {
+      a.mirror = c.n;
+      ct->colio(true, true, false, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
+      check_hip(hipGetLastError(), "k_colio_ct (DCT-I) launch");
       return false;
     }
     const long long nwg = (long long)a.tiles * c.n_outer;

@@ -182,6 +182,10 @@ void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long 
 
 // use the compile-time specialised kernels where a length has them (default on)
 void set_specialised(bool on);
+// DCT-I column passes without the fused Chebyshev recurrence through the
+// direct-load kernel reading the mirrored rows (default on) or through the
+// LDS load pass that builds the extension
+void set_dct_direct(bool on);
 
 
 // out[i] = in[i] over n16 16-byte vectors

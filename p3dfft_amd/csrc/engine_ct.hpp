@@ -97,6 +97,7 @@ struct ColIO {
   int nb;                     // valid columns of this tile
   int ld_lo, ld_hi, ld_shift; // PR: LDS row e <- input row e (e<ld_lo), e-ld_shift (e>=ld_hi), else 0
   int nst, st_lo, st_shift;   // PR: output row e <- FFT row e (e<st_lo) or e+st_shift
+  int mirror;                 // PR, > 0: DCT-I even extension, row e <- input e (e<mirror) or 2(mirror-1)-e
   // table-addressed rows (Off = TabRows): row e of this lane's column at
   // element ltab[e].x + lane_o * ltab[e].y + lane_i of `in` (zero when
   // ltab[e].x < 0), FFT row p stored at stab[p] likewise (skipped when < 0)
@@ -151,7 +152,8 @@ __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
           const cpx<T> u = ((const cpx<T>*)io.in)[el];
           v[m][r] = ok ? u : mk<T>(0, 0);
         } else if constexpr (PR) {
-          const int src = e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1);
+          const int src = io.mirror ? (e < io.mirror ? e : 2 * (io.mirror - 1) - e)
+                                    : (e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1));
           const bool ok = valid && src >= 0;
           const cpx<T> u = ld_at<T, Off>(io.in, ok ? (Off)io.lane_in + (Off)src * (Off)io.sl_in : (Off)0);
           v[m][r] = ok ? u : mk<T>(0, 0);

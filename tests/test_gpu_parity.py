@@ -947,3 +947,27 @@ def test_x_wave_kernels_against_oracle(precision, shape, nxc):
                 assert rel(P.btran_c2r(ref_f, "tff"), ref_b) < TOL[precision], wave
     finally:
         lib.p3dfft_amd_set_x_wave(0)
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("shape,nzc", [((16, 12, 33), None), ((32, 24, 65), 40), ((20, 10, 129), None),
+                                       ((12, 8, 257), 100), ((10, 6, 17), None)])
+def test_dct1_direct_and_lds_passes_against_oracle(precision, shape, nzc):
+    """DCT-I Z passes ('ffc' forward, 'cff' backward) through the direct-load
+    kernel reading the mirrored rows (the default) and through the LDS
+    extension pass, against the oracle; with and without a Z cut (length 17:
+    extension 32, specialised; the others 64 ... 512)."""
+    nx, ny, nz = shape
+    a = np.random.default_rng(nz + nx).standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz, nzc=nzc)
+    ref_f = o.ftran_r2c(a, "ffc")
+    ref_b = o.btran_c2r(ref_f, "cff")[0]
+    lib = p3dfft_amd.load(precision)
+    try:
+        for direct in (1, 0):
+            lib.p3dfft_amd_set_dct_direct(direct)
+            with lib_setup(precision, nx, ny, nz, nzc=nzc) as P:
+                assert rel(P.ftran_r2c(a, "ffc"), ref_f) < TOL[precision], direct
+                assert rel(P.btran_c2r(ref_f, "cff"), ref_b) < TOL[precision], direct
+    finally:
+        lib.p3dfft_amd_set_dct_direct(1)
