@@ -1069,7 +1069,12 @@ __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
   cpx<T>* out = (cpx<T>*)a.out + i0 * a.si_out + (long long)o * a.so_out;
   const int off = (a.kind == COL_DST1) ? 1 : 0;
   constexpr int LU = 8;
-  const int nq = TX * a.nld;
+  const int nq = a.mirror ? 0 : TX * a.nld;
+  if (a.mirror) {
+    // DCT-I: the first stage reads the even extension's mirrored rows from
+    // the input; the other stages, the recurrence and the store use the tile
+    run_fft_in_io<T, TX, NT, unsigned>(lds, tw, col_io<T, TX>(a, (long long)o * a.n_inner + i0, nb), RL{});
+  }
   for (int q0 = threadIdx.x; q0 < nq; q0 += LU * NT) {
     cpx<T> v[LU];
     int dst[LU];
@@ -1091,8 +1096,8 @@ __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
     for (int u = 0; u < LU; u++)
       if (dst[u] >= 0) lds[dst[u]] = v[u];
   }
-  __syncthreads();
-  if (a.kind == COL_DCT1) {
+  if (!a.mirror) __syncthreads();
+  if (a.kind == COL_DCT1 && !a.mirror) {
     const int m = a.L.n;
     for (int q = threadIdx.x; q < TX * (m - a.nld); q += NT) {
       const int r = q / TX, b = q % TX, e = a.nld + r;
@@ -1109,7 +1114,7 @@ __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
     for (int b = threadIdx.x; b < TX; b += NT) lds[b] = mk<T>(0, 0);
     __syncthreads();
   }
-  if (a.kind != COL_COPY) run_fft_ct<T, FWD, TX, TX, NT>(lds, tw, RL{});
+  if (a.kind != COL_COPY && !a.mirror) run_fft_ct<T, FWD, TX, TX, NT>(lds, tw, RL{});
   if constexpr (64 % TX == 0 && NT % 64 == 0) {
     if (a.cheby) cheby_tile<T, TX, NT>(lds, (double*)(g_smem + a.cheby_off), a);
   }
@@ -2351,6 +2356,9 @@ bool launch_col(const ColSpec& c, hipStream_t st) {
       b2 = (size_t)a.cheby_off + (size_t)(ct->NTC / 64) * TX * 4 * sizeof(double);
       if (b2 > kLdsMax) { a.cheby = 0; b2 = bytes; }
     }
+    // DCT-I with the fused recurrence: the first stage reads the mirrored
+    // rows from HBM instead of an LDS load pass and extension copy
+    if (c.kind == COL_DCT1 && g_dct_direct && ld_all && !a.ld_lenfast && !a.st_lenfast && fits) a.mirror = c.n;
     ct->col(fwd, a, (unsigned)nwg, b2, st);
     check_hip(hipGetLastError(), "k_col_ct launch");
     return a.cheby != 0;

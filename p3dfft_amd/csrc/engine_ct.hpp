@@ -238,6 +238,18 @@ __device__ __forceinline__ void run_fft_io(cpx<T>* lds, const typename TwT<T>::t
   mid_and_last<T, FWD, N, B, NT, R0, 0, PR, Off, Rs...>(lds, tw, io);
 }
 
+// Forward FFT of a tile whose first stage is fed from global memory (with
+// the PR row maps, e.g. a DCT-I mirror) and whose result stays in the LDS
+// tile (natural order, element e of column b at lds[e*B + b]).
+template <class T, int B, int NT, class Off, int R0, int... Rs>
+__device__ __forceinline__ void run_fft_in_io(cpx<T>* lds, const typename TwT<T>::type* tw, const ColIO& io,
+                                              RList<R0, Rs...>) {
+  static_assert(sizeof...(Rs) >= 1, "at least two stages");
+  constexpr int N = R0 * RProd<Rs...>::value;
+  stage0_global<T, R0, true, N, B, NT, true, Off>(lds, io);
+  stages_ct<T, true, N, B, B, NT, R0, 0, Rs...>(lds, tw);
+}
+
 // Table-addressed rows (Off = TabRows) with the row tables staged in LDS at
 // `tl` (n int2): the load table before the first stage, the store table
 // after it, so every table read is an LDS read.
