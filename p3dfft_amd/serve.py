@@ -432,9 +432,40 @@ class PoissonBatch:
         L.p3dfft_amd_synchronize()
         return time.perf_counter() - t0
 
+    def capture(self):
+        """Capture one launch of the batch (its grouped launches and any
+        solves run one by one) into a HIP graph on a stream of its own, so
+        the whole batch is one host launch (run_graph replays it)."""
+        L = self.lib
+        self.stream = L.p3dfft_amd_stream_create()
+        g = ctypes.c_void_p()
+        if L.p3dfft_amd_capture_begin(self.stream):
+            raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+        self.launch(self.stream)
+        if L.p3dfft_amd_capture_end(self.stream, ctypes.byref(g)):
+            raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+        self.graph = g.value
+
+    def run_graph(self, reps=1):
+        """`reps` back-to-back replays of the captured batch; returns seconds."""
+        L = self.lib
+        L.p3dfft_amd_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            if L.p3dfft_amd_graph_launch(self.graph, self.stream):
+                raise P3DFFTError(L.p3dfft_amd_last_error().decode())
+        L.p3dfft_amd_stream_synchronize(self.stream)
+        return time.perf_counter() - t0
+
     def close(self):
         L = self.lib
         L.p3dfft_amd_synchronize()
+        if getattr(self, "graph", None):
+            L.p3dfft_amd_graph_destroy(self.graph)
+            self.graph = None
+        if getattr(self, "stream", None):
+            L.p3dfft_amd_stream_destroy(self.stream)
+            self.stream = None
         if self.handle:
             L.p3dfft_poisson_batch_destroy(self.handle)
             self.handle = None

@@ -382,3 +382,34 @@ def test_plain_path_baseline_same_solves():
             assert rel(S.download(i), fused[i]) < 1e-12, (i, shapes[i])
     finally:
         S.close()
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_grouped_batch_captured_graph_bit_identical(precision):
+    """The grouped launches of a batch captured into one HIP graph (the whole
+    batch as one host launch) replay to the eager grouped result bit for
+    bit, three times, with the right-hand sides refilled and eager solves
+    between the replays."""
+    shapes = [(32, 48, 64), (64, 32, 48), (48, 48, 48), (96, 64, 32), (32, 32, 32), (64, 96, 128),
+              (128, 64, 96), (48, 96, 64)]
+    S = serve.PoissonServer(shapes, precision=precision, slots=2)
+    B = None
+    try:
+        idx = list(range(len(shapes)))
+        B = S.batch(idx)
+        S.fill()
+        B.launch()
+        S.lib.p3dfft_amd_synchronize()
+        want = [S.download(i) for i in idx]
+        B.capture()
+        for rep in range(3):
+            S.fill()
+            S.solve_alone(0)  # eager work between the replays
+            S.fill([0])
+            B.run_graph(1)
+            for i in idx:
+                assert np.array_equal(S.download(i), want[i]), (rep, i)
+    finally:
+        if B is not None:
+            B.close()
+        S.close()
