@@ -1433,14 +1433,10 @@ static FDiv make_fdiv(int d) {
 // stage radices: radix 8 as far as possible (two 4s instead of an 8 and a 2),
 // then 3, 5, 7, 11, 13, then any remaining prime (generic stage)
 static std::vector<int> factorize(int n) {
-  // radix 16 where it saves a stage (one LDS exchange fewer) and measured
-  // faster: FP32 384^3 fwd+bwd 0.586 -> 0.569 ms (X passes 5% faster), the
-  // share-mode length-2048 stages 4-10%; FP64 384^3 measured 3% slower with
-  // it, and 256 = 16 x 16 no faster in the served solves
-#ifdef P3DFFT_SINGLE
-  if (n == 384) return {16, 8, 3};
-  if (n == 1024) return {16, 16, 4};
-#endif
+  // 2048 with radix 16: three stages (two LDS exchanges instead of three),
+  // the share-mode length-2048 stages 4-10% faster; radix 16 for 256, 384
+  // and 1024 measured no faster (kernel trace of 384^3 FP32: 505 against
+  // 503 us per fwd+bwd; FP64 384^3 3% slower)
   if (n == 2048) return {16, 16, 8};
   std::vector<int> f;
   int m = n, a = 0;
@@ -1878,11 +1874,7 @@ static const CtInfo* find_ct(const Len1D& L) {
       {32, CtLen<8, 4>::info()},       {48, CtLen<4, 4, 3>::info()},    {64, CtLen<8, 8>::info()},
       {96, CtLen<8, 4, 3>::info()},    {128, CtLen<8, 4, 4>::info()},   {160, CtLen<8, 4, 5>::info()},
       {192, CtLen<8, 8, 3>::info()},   {256, CtLen<8, 8, 4>::info()},   {320, CtLen<8, 8, 5>::info()},
-#ifdef P3DFFT_SINGLE
-      {384, CtLen<16, 8, 3>::info()},  {512, CtLen<8, 8, 8>::info()},   {1024, CtLen<16, 16, 4>::info()},
-#else
       {384, CtLen<8, 4, 4, 3>::info()}, {512, CtLen<8, 8, 8>::info()},  {1024, CtLen<8, 8, 4, 4>::info()},
-#endif
       {2048, CtLen<16, 16, 8>::info()}};
   auto it = tab.find(L.d.n);
   if (it == tab.end()) return nullptr;
