@@ -71,6 +71,10 @@ def test_invalid_dims_rejected_without_gpu():
         P.setup((2, 1), 8, 8, 8)
     with pytest.raises(p3dfft_amd.P3DFFTError, match="Invalid cut dimensions"):
         P.setup((1, 1), 8, 8, 8, nxc=9)
+    # lengths above 2^24 are refused (those up to it run, above 8192 in
+    # several passes)
+    with pytest.raises(p3dfft_amd.P3DFFTError, match="exceeds the supported transform length 16777216"):
+        P.setup((1, 1), 4, 4, (1 << 24) + 1)
 
 
 def test_product_never_imports_the_oracle():
