@@ -936,6 +936,9 @@ int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, 
   const fplan *fx, *fy, *fz;
   int mf;
   if (nx <= 0 || ny <= 0 || nz <= 0 || p1 <= 0 || p2 <= 0 || reps <= 0) return -1;
+  /* every rank needs a non-empty pencil of each split (nx/2+1 and ny over
+   * p1, ny and nz over p2); orc_setup divides by those sizes */
+  if (p1 > nx / 2 + 1 || p1 > ny || p2 > ny || p2 > nz) return -1;
   pl = (orc_plan *)calloc(P, sizeof(orc_plan));
   {
     int dims[2] = {p1, p2}, r;

@@ -509,6 +509,16 @@ def test_ranks_path_reproduces_single_rank(grid):
     assert np.all(tf > 0) and np.all(tb > 0)
 
 
+def test_ranks_path_rejects_grids_without_a_pencil_per_rank():
+    """More ranks along an axis than grid lines is refused (no rank may hold
+    an empty pencil), not run."""
+    a = np.zeros((4, 4, 4))
+    with pytest.raises(ValueError):
+        O.ranks_fwd_bwd(a, 8, 1)
+    with pytest.raises(ValueError):
+        O.ranks_fwd_bwd(a, 1, 8)
+
+
 def test_dims_create_like_the_drivers():
     assert O.dims_create(16) == (4, 4)
     assert O.dims_create(12) == (3, 4)
