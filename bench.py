@@ -610,7 +610,7 @@ def main():
                     help="record no per-stage HIP events (whole-step time only)")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests of the served Poisson leg (0 = skip)")
-    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--batch-reps", type=int, default=20, help="repetitions of the batched leg's batch")
     ap.add_argument("--no-share", action="store_true", help="skip the share-mode (one rank of 2048^3) leg")
     ap.add_argument("--field", choices=["modes", "random"], default="modes",
