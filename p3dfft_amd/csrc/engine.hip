@@ -1399,25 +1399,17 @@ __global__ __launch_bounds__(TX * P) void k_cheby(cpx<T>* o, long long ncol, int
 }
 
 // plain streaming copy (16-byte vectors), used to measure achievable HBM
-// bandwidth and to calibrate traffic counters: each workgroup copies one
-// contiguous chunk of 4 x 256 vectors, every thread issuing its 4 loads
-// before its stores (the fastest of the copy shapes measured on MI355X:
-// 5.78 TB/s over 2 GiB, against 4.62 TB/s for a grid-stride loop with one
-// load in flight per thread, 4.89 TB/s for hipMemcpy device to device)
-__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ in, uint4* __restrict__ out, long long n) {
-  constexpr int U = 4;
-  const long long base = (long long)blockIdx.x * 256 * U + threadIdx.x;
-  uint4 v[U];
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const long long i = base + u * 256;
-    v[u] = in[i < n ? i : 0];
-  }
-#pragma unroll
-  for (int u = 0; u < U; u++) {
-    const long long i = base + u * 256;
-    if (i < n) out[i] = v[u];
-  }
+// bandwidth and to calibrate traffic counters: one vector per thread, the
+// load with the nontemporal hint.  Copy shapes measured on MI355X (GB/s over
+// 0.54 / 1.08 / 2.15 GB): this one 6345 / 6427 / 6462; without the hint
+// 6090 / 6187 / 6161; 4 vectors per thread, loads before stores,
+// 5749 / 5825 / 5734 (6136 / 6203 / 6014 with both hints); 8 vectors per
+// thread 5422 / 5454 / 5393; hipMemcpy device to device 4.89 TB/s.
+typedef unsigned int copy_vec __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy16(const copy_vec* __restrict__ in, copy_vec* __restrict__ out,
+                                                long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = __builtin_nontemporal_load(in + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -2627,9 +2619,9 @@ void launch_cheby(cplx_t* out, long long ncol, int nzc, long long cs, long long 
 
 void launch_copy16(const void* in, void* out, long long n16, hipStream_t st) {
   if (n16 <= 0) return;
-  const long long blocks = (n16 + 1023) / 1024;
+  const long long blocks = (n16 + 255) / 256;
   if (blocks > 0x7fffffffLL) throw Error{"copy too large", -1};
-  k_copy16<<<(unsigned)blocks, 256, 0, st>>>((const uint4*)in, (uint4*)out, n16);
+  k_copy16<<<(unsigned)blocks, 256, 0, st>>>((const copy_vec*)in, (copy_vec*)out, n16);
   check_hip(hipGetLastError(), "k_copy16 launch");
 }
 
