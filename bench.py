@@ -168,17 +168,33 @@ def copy_peak(lib, nbytes=2 << 30, reps=10):
 
 def ranks_timed(f, threads, reps):
     """The oracle's distributed path on the drivers' grid of `threads` ranks,
-    halving the rank count while the grid leaves a rank without a pencil."""
+    halving the rank count while the grid leaves a rank without a pencil.
+    Returns (p1, p2, forward times, backward times, timer slots (ranks,
+    reps, 12))."""
     from oracle import oracle as O
     while True:
         p1, p2 = O.dims_create(threads)
         try:
-            _, _, tf, tb = O.ranks_fwd_bwd(f, p1, p2, reps=reps, spectrum=False, back=False)
-            return p1, p2, tf, tb
+            sl = np.zeros((p1 * p2, reps, 12))
+            _, _, tf, tb = O.ranks_fwd_bwd(f, p1, p2, reps=reps, spectrum=False, back=False, timers=sl)
+            return p1, p2, tf, tb, sl
         except ValueError:
             if threads <= 1:
                 raise
             threads //= 2
+
+
+def timer_slots(sl):
+    """The 12 timer slots of the timed pairs as the reference drivers report
+    them (sample/C/driver_rand.c:271-289): each rank's slots divided by the
+    number of timed loops, then their average, maximum and minimum over the
+    ranks (seconds per fwd+bwd pair)."""
+    per_rank = sl.mean(axis=1)  # (ranks, 12)
+    return {"slots": list(range(1, 13)), "avg": per_rank.mean(axis=0).tolist(),
+            "max": per_rank.max(axis=0).tolist(), "min": per_rank.min(axis=0).tolist(),
+            "meaning": "1-4 fcomm1/fcomm2/bcomm1/bcomm2 exchanges, 5 X r2c, 6 fcomm1 pack+unpack, 7 Y fwd, "
+                       "8 Z fwd, 9 Z bwd + bcomm1 unpack, 10 Y bwd, 11 bcomm2 pack+unpack, 12 X c2r "
+                       "(the fcomm2 pack is untimed, as in the reference)"}
 
 
 def cpu_baseline(n, f, threads, loops=5):
@@ -193,7 +209,7 @@ def cpu_baseline(n, f, threads, loops=5):
     same field; the median pair is the value.  The OpenMP-over-lines 1 x 1
     oracle is timed beside it (one pair) for comparison."""
     from oracle import oracle as O
-    p1, p2, tf, tb = ranks_timed(f, threads, loops + 1)
+    p1, p2, tf, tb, sl = ranks_timed(f, threads, loops + 1)
     tf, tb = tf[1:], tb[1:]
     pairs = [float(a + b) for a, b in zip(tf, tb)]
     med = float(np.median(pairs))
@@ -208,7 +224,8 @@ def cpu_baseline(n, f, threads, loops=5):
                       "%d^3 field in FP64 by the C oracle, timed like the reference driver: one warm-up pair, "
                       "then %d x (ftran 'fft', btran 'tff'); median pair %.2f s (ftran %.2f s, btran %.2f s "
                       "median)" % (p1, p2, n, loops, med, float(np.median(tf)), float(np.median(tb))),
-            "grid": [p1, p2], "pairs_s": pairs, "one_rank_openmp_lines_pair_s": t_lines}
+            "grid": [p1, p2], "pairs_s": pairs, "one_rank_openmp_lines_pair_s": t_lines,
+            "timers_s": timer_slots(sl[:, 1:, :])}
 
 
 def fp64_poisson(f):
@@ -475,11 +492,11 @@ def cpu_baseline_small(n, threads, loops=20):
     like the reference driver: one warm-up pair, then `loops` pairs; the
     median."""
     f = np.random.default_rng(64).random((n, n, n))
-    p1, p2, tf, tb = ranks_timed(f, threads, loops + 1)
+    p1, p2, tf, tb, sl = ranks_timed(f, threads, loops + 1)
     med = float(np.median(tf[1:] + tb[1:]))
     return {"value": 1.0 / med, "unit": "fwd+bwd/s", "cores": p1 * p2, "kind": "port",
             "sample": "64^3 U[0,1) FP64 on a %d x %d grid of ranks, one warm-up pair then %d pairs; median %.5f s"
-                      % (p1, p2, loops, med)}
+                      % (p1, p2, loops, med), "timers_s": timer_slots(sl[:, 1:, :])}
 
 
 def small_leg(n=64, reps=200):

@@ -70,7 +70,7 @@ def lib():
             f.argtypes = [ctypes.POINTER(OrcPlan), _c_dbl_p, _c_dbl_p]
         L.orc_share_counts.argtypes = [ctypes.POINTER(OrcPlan)] + [_c_ll_p] * 8
         L.orc_ranks_fwd_bwd.argtypes = [ctypes.c_int] * 5 + [_c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
-                                                             _c_dbl_p, _c_dbl_p]
+                                                             _c_dbl_p, _c_dbl_p, _c_dbl_p]
         L.orc_ranks_fwd_bwd.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
@@ -88,20 +88,26 @@ def dims_create(nprocs):
     return (a, nprocs // a)
 
 
-def ranks_fwd_bwd(a, p1, p2, reps=1, spectrum=True, back=True):
+def ranks_fwd_bwd(a, p1, p2, reps=1, spectrum=True, back=True, timers=None):
     """The distributed CPU path ('fft' then 'tff', no cuts) on a p1 x p2
     grid with the ranks as threads.  a: real field (nz, ny, nx).  Returns
     (spectrum (nz, ny, nx/2+1) or None, back (nz, ny, nx) or None, forward
-    times, backward times) in seconds."""
+    times, backward times) in seconds.  timers: optional float64 array of
+    shape (p1*p2, reps, 12), filled with each rank's 12 timer slots per
+    repetition (the reference's timers(1..12) meaning, seconds)."""
     a = np.ascontiguousarray(a, dtype=np.float64)
     nz, ny, nx = a.shape
     spec = np.zeros((nz, ny, nx // 2 + 1), dtype=np.complex128) if spectrum else None
     out = np.zeros_like(a) if back else None
     tf = np.zeros(reps)
     tb = np.zeros(reps)
+    if timers is not None:
+        if timers.shape != (p1 * p2, reps, 12) or timers.dtype != np.float64:
+            raise ValueError("timers must be float64 of shape (p1*p2, reps, 12)")
     rc = lib().orc_ranks_fwd_bwd(nx, ny, nz, p1, p2, _dp(a),
                                  _dp(spec.view(np.float64)) if spectrum else None,
-                                 _dp(out) if back else None, reps, _dp(tf), _dp(tb))
+                                 _dp(out) if back else None, reps, _dp(tf), _dp(tb),
+                                 _dp(timers) if timers is not None else None)
     if rc:
         raise ValueError("oracle ranks_fwd_bwd failed (%d)" % rc)
     return spec, out, tf, tb

@@ -928,8 +928,17 @@ static void ranks_line(const fplan *fp, double *base, long long stride, int sign
   for (j = 0; j < fp->n; j++) { base[2 * j * stride] = y[2 * j]; base[2 * j * stride + 1] = y[2 * j + 1]; }
 }
 
+/* wall clock of the rank threads (0 without OpenMP: no stage times) */
+static double ranks_now(void) {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0.0;
+#endif
+}
+
 int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, double *spec, double *out,
-                      int reps, double *t_fwd, double *t_bwd) {
+                      int reps, double *t_fwd, double *t_bwd, double *slots) {
   int P = p1 * p2, ok = 1;
   orc_plan *pl;
   double **xr, **b2, **s1, **r1, **yp, **s2, **zp;
@@ -997,18 +1006,33 @@ int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, 
 #pragma omp barrier
 #endif
     for (it = 0; it < reps; it++) {
-      double t0 = 0;
+      double t0 = 0, ts;
+      /* this rank's 12 timer slots of this repetition, as the reference
+       * fills timers(1..12) (module.F90:106): 1-4 the fcomm1 / fcomm2 /
+       * bcomm1 / bcomm2 exchanges, 5 X r2c, 6 fcomm1 pack and unpack, 7 Y
+       * forward, 8 Z forward, 9 Z backward and the bcomm1 unpack, 10 Y
+       * backward, 11 bcomm2 pack and unpack, 12 X c2r (ftran.F90:528-532,
+       * 579-584, 600-680; btran.F90:519-592, 608-624, 639-642; fcomm1.F90:
+       * 236-322; bcomm2.F90:228-315); the fcomm2 pack is untimed there
+       * (fcomm2.F90:265-271) and here */
+      double *sl = slots ? slots + ((long long)r * reps + it) * 12 : NULL;
+      double sink[12];
+      if (!sl) sl = sink;
+      for (i = 0; i < 12; i++) sl[i] = 0.0;
 #ifdef _OPENMP
 #pragma omp barrier
       if (r == 0) t0 = omp_get_wtime();
 #endif
       /* X r2c */
+      ts = ranks_now();
       for (l = 0; l < nrows; l++) {
         for (j = 0; j < nx; j++) { c[2 * j] = xr[r][l * nx + j]; c[2 * j + 1] = 0.0; }
         fft_line(fx, c, 1, y, -1, tmp);
         memcpy(b2[r] + 2 * l * p->nxhp, y, sizeof(double) * 2 * p->nxhp);
       }
+      sl[4] += ranks_now() - ts; ts = ranks_now();
       orc_fcomm1_pack(p, b2[r], s1[r]);
+      sl[5] += ranks_now() - ts; ts = ranks_now();
 #ifdef _OPENMP
 #pragma omp barrier
 #endif
@@ -1018,13 +1042,17 @@ int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, 
         const long long off = (long long)(p->iistart - 1) * pp->jisize * pp->kjsize;
         memcpy(r1[r] + 2 * fr_st[i], s1[jp * p1 + i] + 2 * off, sizeof(double) * 2 * fr_ct[i]);
       }
+      sl[0] += ranks_now() - ts; ts = ranks_now();
       orc_fcomm1_unpack(p, r1[r], yp[r]);
+      sl[5] += ranks_now() - ts; ts = ranks_now();
       /* Y c2c */
       for (q = 0; q < (long long)p->iisize * p->kjsize; q++) {
         const long long xx = q % p->iisize, zz = q / p->iisize;
         ranks_line(fy, yp[r] + 2 * (xx + zz * (long long)p->iisize * ny), p->iisize, -1, y, tmp);
       }
+      sl[6] += ranks_now() - ts;
       orc_fcomm2_pack(p, yp[r], s2[r]);
+      ts = ranks_now();
 #ifdef _OPENMP
 #pragma omp barrier
 #endif
@@ -1035,9 +1063,11 @@ int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, 
         const long long off = (long long)(p->jjstart - 1) * pp->iisize * pp->kjsize;
         memcpy(zp[r] + 2 * kr_st[j], s2[j * p1 + ip] + 2 * off, sizeof(double) * 2 * kr_ct[j]);
       }
+      sl[1] += ranks_now() - ts; ts = ranks_now();
       /* Z c2c */
       for (q = 0; q < (long long)p->iisize * p->jjsize; q++)
         ranks_line(fz, zp[r] + 2 * q, (long long)p->iisize * p->jjsize, -1, y, tmp);
+      sl[7] += ranks_now() - ts;
 #ifdef _OPENMP
 #pragma omp barrier
       if (r == 0) { t_fwd[it] = omp_get_wtime() - t0; t0 = omp_get_wtime(); }
@@ -1054,8 +1084,10 @@ int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, 
 #endif
       }
       /* backward: Z c2c */
+      ts = ranks_now();
       for (q = 0; q < (long long)p->iisize * p->jjsize; q++)
         ranks_line(fz, zp[r] + 2 * q, (long long)p->iisize * p->jjsize, +1, y, tmp);
+      sl[8] += ranks_now() - ts; ts = ranks_now();
 #ifdef _OPENMP
 #pragma omp barrier
 #endif
@@ -1067,13 +1099,17 @@ int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, 
         const long long off = (long long)(p->kjstart - 1) * pp->iisize * pp->jjsize;
         memcpy(s2[r] + 2 * ks_st[j], zp[j * p1 + ip] + 2 * off, sizeof(double) * 2 * ks_ct[j]);
       }
+      sl[2] += ranks_now() - ts; ts = ranks_now();
       orc_bcomm1_unpack(p, s2[r], yp[r]);
+      sl[8] += ranks_now() - ts; ts = ranks_now();
       /* Y c2c */
       for (q = 0; q < (long long)p->iisize * p->kjsize; q++) {
         const long long xx = q % p->iisize, zz = q / p->iisize;
         ranks_line(fy, yp[r] + 2 * (xx + zz * (long long)p->iisize * ny), p->iisize, +1, y, tmp);
       }
+      sl[9] += ranks_now() - ts; ts = ranks_now();
       orc_bcomm2_pack(p, yp[r], r1[r]);
+      sl[10] += ranks_now() - ts; ts = ranks_now();
 #ifdef _OPENMP
 #pragma omp barrier
 #endif
@@ -1084,9 +1120,12 @@ int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, 
         const long long off = (long long)(p->jistart - 1) * pp->iisize * pp->kjsize;
         memcpy(s1[r] + 2 * fs_st[i], r1[jp * p1 + i] + 2 * off, sizeof(double) * 2 * fs_ct[i]);
       }
+      sl[3] += ranks_now() - ts; ts = ranks_now();
       orc_bcomm2_unpack(p, s1[r], b2[r]);
+      sl[10] += ranks_now() - ts; ts = ranks_now();
       /* X c2r */
       for (l = 0; l < nrows; l++) c2r_line(fx, b2[r] + 2 * l * p->nxhp, 1, xr[r] + l * nx, 1, c, y, tmp);
+      sl[11] += ranks_now() - ts;
 #ifdef _OPENMP
 #pragma omp barrier
       if (r == 0) t_bwd[it] = omp_get_wtime() - t0;

@@ -107,9 +107,11 @@ void orc_share_counts(const orc_plan *p, long long *fsnd_st, long long *fsnd_ct,
 /* ops 'fft' / 'tff' without cuts; in: global real (nx,ny,nz); spec
  * (optional): global spectrum (nxhp,ny,nz) after the last forward; out
  * (optional): global real after the last backward; t_fwd / t_bwd: reps wall
- * times (s).  Returns 0 or -1. */
+ * times (s); slots (optional): p1*p2*reps*12 values, rank r's 12 timer
+ * slots of repetition it at ((r*reps)+it)*12 (s, the reference's timers(1..12)
+ * meaning).  Returns 0 or -1. */
 int orc_ranks_fwd_bwd(int nx, int ny, int nz, int p1, int p2, const double *in, double *spec, double *out,
-                      int reps, double *t_fwd, double *t_bwd);
+                      int reps, double *t_fwd, double *t_bwd, double *slots);
 
 /* number of OpenMP threads the oracle uses (0 = leave default) */
 void orc_set_threads(int n);

@@ -509,6 +509,23 @@ def test_ranks_path_reproduces_single_rank(grid):
     assert np.all(tf > 0) and np.all(tb > 0)
 
 
+def test_ranks_path_timer_slots():
+    """The 12 per-rank timer slots of the distributed path: every slot the
+    reference fills on a 2 x 2 grid is timed (slots 1-12; the exchanges 1-4
+    include the barrier wait) and non-negative, and the slots of one
+    repetition add up to no more than that repetition's wall time."""
+    a = np.random.default_rng(3).standard_normal((40, 36, 32))
+    reps = 3
+    sl = np.full((4, reps, 12), -1.0)
+    _, _, tf, tb = O.ranks_fwd_bwd(a, 2, 2, reps=reps, spectrum=False, back=False, timers=sl)
+    assert np.all(sl >= 0)
+    # compute slots on every rank and every repetition
+    assert np.all(sl[:, :, [4, 6, 7, 8, 9, 11]] > 0)
+    assert np.all(sl.sum(axis=2)[0] <= (tf + tb) * 1.05 + 1e-4)
+    with pytest.raises(ValueError):
+        O.ranks_fwd_bwd(a, 2, 2, reps=reps, timers=np.zeros((4, reps, 11)))
+
+
 def test_ranks_path_rejects_grids_without_a_pencil_per_rank():
     """More ranks along an axis than grid lines is refused (no rank may hold
     an empty pencil), not run."""
