@@ -971,3 +971,27 @@ def test_dct1_direct_and_lds_passes_against_oracle(precision, shape, nzc):
                 assert rel(P.btran_c2r(ref_f, "cff"), ref_b) < TOL[precision], direct
     finally:
         lib.p3dfft_amd_set_dct_direct(1)
+
+
+@pytest.mark.parametrize("nvec", [1, 255, 257, 1000003])
+def test_stream_copy_exact(nvec):
+    """The streaming copy kernel (the copy-bandwidth reference of the bench
+    and the traffic-counter calibration) copies every byte, also when the
+    last workgroup is partial, and leaves the bytes after the copy alone;
+    sizes that are not a multiple of 16 bytes are refused."""
+    lib = p3dfft_amd.load(4)
+    nbytes = 16 * nvec
+    src = np.random.default_rng(nvec).integers(0, 256, nbytes + 64, dtype=np.uint8)
+    a, b = p3dfft_amd.DeviceBuffer(lib, nbytes + 64), p3dfft_amd.DeviceBuffer(lib, nbytes + 64)
+    try:
+        a.upload(src)
+        b.upload(np.zeros(nbytes + 64, dtype=np.uint8))
+        assert lib.p3dfft_amd_stream_copy(a.ptr, b.ptr, nbytes, None) == 0
+        assert lib.p3dfft_amd_synchronize() == 0
+        got = b.download(nbytes + 64, np.uint8)
+        assert np.array_equal(got[:nbytes], src[:nbytes])
+        assert not got[nbytes:].any()
+        assert lib.p3dfft_amd_stream_copy(a.ptr, b.ptr, nbytes + 8, None) == -1
+    finally:
+        a.free()
+        b.free()
