@@ -421,8 +421,8 @@ __device__ __forceinline__ void x_r2c_ct_tile(cpx<T>* lds, const typename TwT<T>
   const unsigned so = (unsigned)(a.so_row * sizeof(cpx<T>));
   const T h = (T)0.5;
   const bool full = nr == 2 * B;  // uniform: no per-store row guard
-  for (int q = threadIdx.x; q < B * a.nxo; q += NT) {
-    const int b = fdiv(q, a.dnxo), k = q - b * a.nxo;
+  // separation of mode k of pair b and its stores
+  auto put = [&](int b, int k) {
     const cpx<T> zk = lds[k * BP + b], zn = lds[(k == 0 ? 0 : N - k) * BP + b];
     const unsigned o = (unsigned)(2 * b) * so + (unsigned)k * (unsigned)sizeof(cpx<T>);
     const cpx<T> ya = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h), yb = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
@@ -438,6 +438,18 @@ __device__ __forceinline__ void x_r2c_ct_tile(cpx<T>* lds, const typename TwT<T>
     } else {
       if (2 * b < nr) *(cpx<T>*)(out + o) = ya;
       if (2 * b + 1 < nr) *(cpx<T>*)(out + o + so) = yb;
+    }
+  };
+  constexpr int H = N / 2;
+  if (!XT && (N % 2) == 0 && a.nxo == H + 1) {
+    // every mode stored (no X cut): modes k < N/2 as item b*(N/2) + k
+    // (compile-time divisor), the Nyquist mode of pair b by thread b
+    for (int q = threadIdx.x; q < B * H; q += NT) put(q / H, q % H);
+    if (threadIdx.x < B) put(threadIdx.x, H);
+  } else {
+    for (int q = threadIdx.x; q < B * a.nxo; q += NT) {
+      const int b = fdiv(q, a.dnxo);
+      put(b, q - b * a.nxo);
     }
   }
 }
@@ -461,7 +473,46 @@ __device__ __forceinline__ void x_c2r_ct_tile(cpx<T>* lds, const typename TwT<T>
   constexpr int H = N / 2;
   const char* in = (const char*)a.in + 2 * pair0 * a.si_row * (long long)sizeof(cpx<T>);
   const unsigned si = (unsigned)(a.si_row * sizeof(cpx<T>));
-  {
+  if (!XT && (N % 2) == 0 && a.nxo == H + 1) {
+    // every mode stored (no X cut): the modes k < N/2 of pair b as item
+    // b*(N/2) + k (compile-time divisor: shifts for the power-of-two
+    // lengths, whole iterations for every thread), the Nyquist mode of pair b
+    // by thread b; values and positions as in the general loop below
+    constexpr int LH = (B * H + NT - 1) / NT;
+    cpx<T> va[LH], vb[LH];
+#pragma unroll
+    for (int i = 0; i < LH; i++) {
+      const int q = threadIdx.x + i * NT;
+      const int b = q / H, k = q % H;
+      const bool ok = ((B * H) % NT == 0) || q < B * H;
+      const bool oka = ok && 2 * b < nr, okb = ok && 2 * b + 1 < nr;
+      const unsigned o = (unsigned)(2 * b) * si + (unsigned)k * (unsigned)sizeof(cpx<T>);
+      const cpx<T> ua = *(const cpx<T>*)(in + (oka ? o : 0u));
+      const cpx<T> ub = *(const cpx<T>*)(in + (okb ? o + si : 0u));
+      va[i] = oka ? ua : mk<T>(0, 0);
+      vb[i] = okb ? ub : mk<T>(0, 0);
+    }
+    cpx<T> na = mk<T>(0, 0), nb = mk<T>(0, 0);
+    if (threadIdx.x < B) {
+      const int b = threadIdx.x;
+      const unsigned o = (unsigned)(2 * b) * si + (unsigned)H * (unsigned)sizeof(cpx<T>);
+      if (2 * b < nr) na = *(const cpx<T>*)(in + o);
+      if (2 * b + 1 < nr) nb = *(const cpx<T>*)(in + o + si);
+    }
+#pragma unroll
+    for (int i = 0; i < LH; i++) {
+      const int q = threadIdx.x + i * NT;
+      if (((B * H) % NT == 0) || q < B * H) {
+        const int b = q / H, k = q % H;
+        cpx<T> A = va[i], Bv = vb[i];
+        if (k == 0) { A.y = 0; Bv.y = 0; }
+        lds[k * BP + b] = mk<T>(A.x - Bv.y, A.y + Bv.x);
+        lds[(k == 0 ? 0 : N - k) * BP + b] = mk<T>(A.x + Bv.y, Bv.x - A.y);
+      }
+    }
+    // Nyquist: Im of A and B ignored, Z[N/2] = Re A + i Re B
+    if (threadIdx.x < B) lds[H * BP + threadIdx.x] = mk<T>(na.x, nb.x);
+  } else {
     // each input element k <= N/2 is loaded once and gives Z[k] and Z[N-k]
     // (Z = A + iB of the two Hermitian half spectra)
     constexpr int K = H + 1;
