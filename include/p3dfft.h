@@ -284,6 +284,12 @@ void p3dfft_amd_set_x_wave(int on);
  * extension in LDS first.  No reference counterpart; results agree to
  * rounding. */
 void p3dfft_amd_set_dct_direct(int on);
+/* Column passes of the lengths whose tile takes more than half the LDS (one
+ * workgroup per CU: 2048 points): 1 (default) through a persistent kernel
+ * (one workgroup per CU looping over tiles, the next tile's loads issued
+ * while the current one computes), 0 one workgroup per tile.
+ * No reference counterpart; results agree to rounding. */
+void p3dfft_amd_set_col_pipe(int on);
 /* 1 (default): a forward without Y/Z cuts whose output does not overlap its
  * input runs X r2c straight into the output and the Y pass in place there
  * (the Z pass then reads lines the Y pass left in the last-level cache);

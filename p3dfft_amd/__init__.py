@@ -91,6 +91,7 @@ def load(precision=8):
     L.p3dfft_amd_set_plane_kernels.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_x_wave.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_dct_direct.argtypes = [ctypes.c_int]
+    L.p3dfft_amd_set_col_pipe.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_x_into_out.argtypes = [ctypes.c_int]
     L.p3dfft_amd_set_plan_mode.argtypes = [ctypes.c_int]
     L.p3dfft_amd_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.POINTER(_vp)]

@@ -1257,6 +1257,7 @@ void p3dfft_amd_set_flat_order(int mode) { set_flat_order(mode); }
 void p3dfft_amd_set_plane_kernels(int mode) { set_plane_kernels(mode); }
 void p3dfft_amd_set_x_wave(int on) { set_x_wave(on != 0); }
 void p3dfft_amd_set_dct_direct(int on) { set_dct_direct(on != 0); }
+void p3dfft_amd_set_col_pipe(int on) { set_col_pipe(on); }
 void p3dfft_amd_set_x_into_out(int on) { g_x_into_out = (on != 0); }
 
 // 0 (default): the default plan of every length; 1: from the next setup on,

@@ -1022,6 +1022,51 @@ __global__ __launch_bounds__(NT, W) void k_colio_ct(ColArgs a) {
     run_fft_io<T, FWD, TX, NT, PR, Off>(lds, tw, io, RL{});
 }
 
+// Persistent form of k_colio_ct (unpruned rows, 32-bit offsets or the
+// table-addressed rows) for tiles that take more than half the LDS, i.e. one
+// workgroup per CU: gridDim.x workgroups, the 8 XCDs (workgroup w on XCD
+// w mod 8) each owning a contiguous range of tiles that its workgroups
+// share out round robin (a.xcd; else workgroup w takes tiles w, w + G, ...),
+// and each workgroup keeping its next tile's loads in flight while the
+// current tile computes.  The row tables (TabRows) are the
+// same for every tile and stay in LDS, load table then store table, at
+// tab_off.
+template <class T, bool FWD, int TX, int NT, class RL, int W, bool TWL, class Off>
+__global__ __launch_bounds__(NT, W) void k_colio_pipe(ColArgs a) {
+  constexpr int N = RL_N<RL>::value;
+  constexpr bool TB = std::is_same<Off, TabRows>::value;
+  cpx<T>* lds = (cpx<T>*)g_smem;
+  // the twiddle staging is completed by the barrier at the end of the first stage
+  const auto* tw = ct_twiddles<T, NT, TWL>(a.tw, a.tw_off, a.L.twcount);
+  const long long ncol = (long long)a.n_inner * a.n_outer;
+  const int ntiles = (int)((ncol + TX - 1) / TX);
+  const int G = (int)gridDim.x, x = (int)(blockIdx.x & 7), slot = (int)(blockIdx.x >> 3);
+  const int nslot = G / 8 + (x < G % 8 ? 1 : 0);
+  const int q = ntiles / 8, rm = ntiles % 8;
+  const int lo = x * q + (x < rm ? x : rm), cnt = q + (x < rm ? 1 : 0);
+  int2* tl = (int2*)(g_smem + a.tab_off);
+  if constexpr (TB) {
+    for (int i = threadIdx.x; i < N; i += NT) {
+      tl[i] = a.ltab[i];
+      tl[N + i] = a.stab[i];
+    }
+    __syncthreads();
+  }
+  auto make_io = [&](int t) -> ColIO {
+    const long long c0 = (long long)(a.rev ? ntiles - 1 - t : t) * TX;
+    const int nb = (int)(ncol - c0 < TX ? ncol - c0 : TX);
+    ColIO io = col_io<T, TX>(a, c0, nb);
+    if constexpr (TB) {
+      io.ltab = tl;
+      io.stab = tl + N;
+    }
+    return io;
+  };
+  if (a.xcd) run_fft_io_pipe<T, FWD, TX, NT, false, Off>(lds, tw, lo + slot, lo + cnt, nslot, make_io, RL{});
+  // dispatch order: the tiles running at once form one contiguous range
+  else run_fft_io_pipe<T, FWD, TX, NT, false, Off>(lds, tw, (int)blockIdx.x, ntiles, G, make_io, RL{});
+}
+
 // Chebyshev derivative recurrence (build/ftran.F90:408-451) on the output
 // rows of an LDS column tile: output row e of column b sits in LDS row
 // p(e) = e (e < st_lo) or e + st_shift.  As in k_cheby, two interleaved
@@ -1770,9 +1815,12 @@ struct CtInfo {
   int radix[kMaxStages];
   bool twl_col, twl_x;  // twiddles staged in LDS (column / X kernels)
   bool twl_tab;         // the same for the table-addressed column kernel
+  bool pipe;            // the column tile takes more than half the LDS: k_colio_pipe exists
+  bool twl_pipe_tab;    // twiddles in LDS in the table-addressed k_colio_pipe
   long long twb;        // twiddle table bytes
   void (*col)(bool, const ColArgs&, unsigned, size_t, hipStream_t);
   void (*colio)(bool, bool, bool, const ColArgs&, unsigned, size_t, hipStream_t);
+  void (*colpipe)(bool, bool, const ColArgs&, unsigned, size_t, hipStream_t);
   void (*xr2c)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*xc2r)(const XArgs&, unsigned, size_t, hipStream_t);
   void (*zsolve)(const ColArgs&, double, int, int, unsigned, size_t, hipStream_t);
@@ -1813,6 +1861,10 @@ struct CtLen {
   // (read from global memory instead, the tables cost more than the twiddles)
   static constexpr bool TWLT = TWLC && round16(DC + TWB) + N * (long long)sizeof(int2) <= (long long)kLdsMax;
   static constexpr bool TWLX = TWB > 0 && DX + TWB <= kCtLdsTw;
+  // persistent column kernel: where the tile alone allows one workgroup per
+  // CU; the table-addressed form keeps both row tables (2 N int2) in LDS
+  static constexpr bool PIPE = DC > kCtLdsTw;
+  static constexpr bool TWLP = TWLC && round16(DC + TWB) + 2 * N * (long long)sizeof(int2) <= (long long)kLdsMax;
   // waves per SIMD the LDS footprint allows: the register budget of each kernel
   static constexpr int WC = ct_waves(DC + (TWLC ? TWB : 0), NTC);
   static constexpr int WX = ct_waves(DX + (TWLX ? TWB : 0), NTX);
@@ -1848,6 +1900,18 @@ struct CtLen {
       else go(k_colio_ct<real_t, false, TX, NTC, RL, WC, TWLC, false, U32>, g, NTC, bytes, s, a);
     }
   }
+  // tab: table-addressed rows (a.ltab), else unpruned rows at 32-bit offsets
+  static void colpipe(bool fwd, bool tab, const ColArgs& a, unsigned g, size_t bytes, hipStream_t s) {
+    if constexpr (PIPE) {
+      if (tab) {
+        if (fwd) go(k_colio_pipe<real_t, true, TX, NTC, RL, WC, TWLP, TabRows>, g, NTC, bytes, s, a);
+        else go(k_colio_pipe<real_t, false, TX, NTC, RL, WC, TWLP, TabRows>, g, NTC, bytes, s, a);
+      } else {
+        if (fwd) go(k_colio_pipe<real_t, true, TX, NTC, RL, WC, TWLC, unsigned>, g, NTC, bytes, s, a);
+        else go(k_colio_pipe<real_t, false, TX, NTC, RL, WC, TWLC, unsigned>, g, NTC, bytes, s, a);
+      }
+    }
+  }
   static void xr2c(const XArgs& a, unsigned g, size_t bytes, hipStream_t s) {
     if (a.xt) gox(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX, true>, g, NTX, bytes, s, a);
     else gox(k_x_r2c_ct<real_t, BX, NTX, RL, WX, TWLX, false>, g, NTX, bytes, s, a);
@@ -1869,6 +1933,10 @@ struct CtLen {
     allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, true, unsigned>);
     allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, unsigned long long>);
     allow_big_lds(k_colio_ct<real_t, FWD, TX, NTC, RL, WC, TWLC, false, unsigned>);
+    if constexpr (PIPE) {
+      allow_big_lds(k_colio_pipe<real_t, FWD, TX, NTC, RL, WC, TWLP, TabRows>);
+      allow_big_lds(k_colio_pipe<real_t, FWD, TX, NTC, RL, WC, TWLC, unsigned>);
+    }
   }
   static void xgrp(bool r2c, const XArgs& a, const GrpArgs& ga, unsigned g, size_t bytes, hipStream_t s) {
     if (r2c) k_x_grp<real_t, true, BX, NTX, RL, WX, TWLX><<<g, NTX, bytes, s>>>(a, ga);
@@ -1903,7 +1971,8 @@ struct CtLen {
     c.nr = (int)sizeof...(Rs);
     for (int i = 0; i < c.nr; i++) c.radix[i] = rs[i];
     c.twl_col = TWLC; c.twl_x = TWLX; c.twl_tab = TWLT; c.twb = TWB;
-    c.col = &col; c.colio = &colio; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve; c.prep = &prep;
+    c.pipe = PIPE; c.twl_pipe_tab = TWLP;
+    c.col = &col; c.colio = &colio; c.colpipe = &colpipe; c.xr2c = &xr2c; c.xc2r = &xc2r; c.zsolve = &zsolve; c.prep = &prep;
     c.xgrp = &xgrp; c.cgrp = &cgrp; c.zgrp = &zgrp;
     return c;
   }
@@ -2272,6 +2341,22 @@ static bool col_fits32(const ColSpec& c, int TX, int rows) {
   return span(c.si_in, c.sl_in, c.so_in) < (1LL << 32) && span(c.si_out, c.sl_out, c.so_out) < (1LL << 32);
 }
 
+// Persistent column kernels (k_colio_pipe) for the lengths whose tile takes
+// more than half the LDS (2048 points): on (1, default) or off (0).  Measured
+// on MI355X, share-mode rank of 2048^3 on 16x16, FP32, off -> on (us):
+// ftran_y 101.0 / 100.3 -> 91.4 / 93.2, btran_y 108.6 / 109.1 -> 100.3 / 99.3,
+// ftran_z 89.8 / 82.7 -> 80.9 / 81.1; fwd+bwd 0.569 / 0.561 -> 0.533 / 0.547 ms.
+static int g_col_pipe = 1;
+void set_col_pipe(int on) { g_col_pipe = on ? 1 : 0; }
+
+// workgroups of a persistent launch: one per CU of the current device
+static unsigned pipe_grid(long long ntiles) {
+  int dev = 0, ncu = 0;
+  check_hip(hipGetDevice(&dev), "hipGetDevice");
+  check_hip(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+  return (unsigned)std::max<long long>(1, std::min<long long>(ntiles, std::max(ncu, 1)));
+}
+
 bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStream_t st) {
   if (c.n_inner <= 0 || c.n_outer <= 0) return true;
   if (c.kind != COL_C2C || !g_use_ct || c.n > max_len()) return false;
@@ -2294,6 +2379,16 @@ bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStr
   a.tiles = (c.n_inner + TX - 1) / TX;
   a.L = L.d;
   a.tw = L.tw;
+  if (g_col_pipe && ct->pipe) {
+    size_t pb = ct_lds((size_t)TX * c.n * sizeof(cplx_t), ct->twl_pipe_tab, *ct, &a.tw_lds, &a.tw_off);
+    a.tab_off = (int)round16((long long)pb);
+    pb = (size_t)a.tab_off + 2 * (size_t)c.n * sizeof(int2);
+    if (pb <= kLdsMax) {
+      ct->colpipe(c.fwd, true, a, pipe_grid((ncol + TX - 1) / TX), pb, st);
+      check_hip(hipGetLastError(), "k_colio_pipe (table) launch");
+      return true;
+    }
+  }
   size_t bytes = ct_lds((size_t)TX * c.n * sizeof(cplx_t), ct->twl_tab, *ct, &a.tw_lds, &a.tw_off);
   a.tab_off = (int)round16((long long)bytes);
   bytes = (size_t)a.tab_off + (size_t)c.n * sizeof(int2);
@@ -2363,6 +2458,11 @@ bool launch_col(const ColSpec& c, hipStream_t st) {
     const bool st_id = c.nst == nfft && (c.st_lo >= nfft || c.st_shift == 0);
     const bool pr = !(ld_id && st_id);
     const bool fits = col_fits32(c, TX, rows) && !g_wide_offsets;
+    if (c.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast && fits && !pr && g_col_pipe && ct->pipe) {
+      ct->colpipe(fwd, false, a, pipe_grid((ncol + TX - 1) / TX), bytes, st);
+      check_hip(hipGetLastError(), "k_colio_pipe launch");
+      return false;
+    }
     if (c.kind == COL_C2C && !a.ld_lenfast && !a.st_lenfast && (fits || !pr)) {
       ct->colio(fwd, pr, !fits, a, (unsigned)((ncol + TX - 1) / TX), bytes, st);
       check_hip(hipGetLastError(), "k_colio_ct launch");

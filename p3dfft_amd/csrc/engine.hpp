@@ -186,6 +186,7 @@ void set_specialised(bool on);
 // direct-load kernel reading the mirrored rows (default on) or through the
 // LDS load pass that builds the extension
 void set_dct_direct(bool on);
+void set_col_pipe(int on);
 
 
 // out[i] = in[i] over n16 16-byte vectors

@@ -126,15 +126,20 @@ __device__ __forceinline__ void st_at(char* base, Off off, cpx<T> v) {
   *(cpx<T>*)(base + off) = v;
 }
 
-// first stage: loads of one tile into registers, butterflies (no twiddles),
-// results into the LDS tile
-template <class T, int R, bool FWD, int N, int B, int NT, bool PR, class Off>
-__device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
+// the first stage's values of one tile, held in registers between the loads
+// and the butterflies
+template <class T, int R, int N, int B, int NT>
+struct Stage0Regs {
+  cpx<T> v[Stage0<R, N, B, NT>::IT][R];
+};
+
+// first stage, loads: the tile's values of every butterfly of this thread
+template <class T, int R, int N, int B, int NT, bool PR, class Off>
+__device__ __forceinline__ void stage0_load(Stage0Regs<T, R, N, B, NT>& rg, const ColIO& io) {
   static_assert(NT % B == 0, "one column per thread");
   constexpr bool TB = std::is_same<Off, TabRows>::value;
   typedef Stage0<R, N, B, NT> S;
   const bool valid = (int)(threadIdx.x % B) < io.nb;
-  cpx<T> v[S::IT][R];
 #pragma unroll
   for (int m = 0; m < S::IT; m++) {
     const int it = threadIdx.x + m * NT;
@@ -150,31 +155,46 @@ __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
           const bool ok = valid && t.x >= 0;
           const unsigned el = ok ? (unsigned)t.x + io.lane_o * (unsigned)t.y + io.lane_i : 0u;
           const cpx<T> u = ((const cpx<T>*)io.in)[el];
-          v[m][r] = ok ? u : mk<T>(0, 0);
+          rg.v[m][r] = ok ? u : mk<T>(0, 0);
         } else if constexpr (PR) {
           const int src = io.mirror ? (e < io.mirror ? e : 2 * (io.mirror - 1) - e)
                                     : (e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1));
           const bool ok = valid && src >= 0;
           const cpx<T> u = ld_at<T, Off>(io.in, ok ? (Off)io.lane_in + (Off)src * (Off)io.sl_in : (Off)0);
-          v[m][r] = ok ? u : mk<T>(0, 0);
+          rg.v[m][r] = ok ? u : mk<T>(0, 0);
         } else {
           const cpx<T> u = ld_at<T, Off>(io.in, valid ? (Off)io.lane_in + (Off)e * (Off)io.sl_in : (Off)0);
-          v[m][r] = valid ? u : mk<T>(0, 0);
+          rg.v[m][r] = valid ? u : mk<T>(0, 0);
         }
       }
     }
   }
+}
+
+// first stage, butterflies (no twiddles) and results into the LDS tile
+template <class T, int R, bool FWD, int N, int B, int NT>
+__device__ __forceinline__ void stage0_put(cpx<T>* lds, Stage0Regs<T, R, N, B, NT>& rg) {
+  typedef Stage0<R, N, B, NT> S;
 #pragma unroll
   for (int m = 0; m < S::IT; m++) {
     const int it = threadIdx.x + m * NT;
     if ((S::total % NT) == 0 || it < S::total) {
       const int j = it / B, b = it % B;
-      dft<R, FWD>(v[m]);
+      dft<R, FWD>(rg.v[m]);
 #pragma unroll
-      for (int r = 0; r < R; r++) lds[(j * R + r) * B + b] = v[m][r];
+      for (int r = 0; r < R; r++) lds[(j * R + r) * B + b] = rg.v[m][r];
     }
   }
   __syncthreads();
+}
+
+// first stage: loads of one tile into registers, butterflies (no twiddles),
+// results into the LDS tile
+template <class T, int R, bool FWD, int N, int B, int NT, bool PR, class Off>
+__device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
+  Stage0Regs<T, R, N, B, NT> rg;
+  stage0_load<T, R, N, B, NT, PR, Off>(rg, io);
+  stage0_put<T, R, FWD, N, B, NT>(lds, rg);
 }
 
 // last stage: Ns * R == N, so every butterfly index j < Ns and k == j
@@ -236,6 +256,32 @@ __device__ __forceinline__ void run_fft_io(cpx<T>* lds, const typename TwT<T>::t
   constexpr int N = R0 * RProd<Rs...>::value;
   stage0_global<T, R0, FWD, N, B, NT, PR, Off>(lds, io);
   mid_and_last<T, FWD, N, B, NT, R0, 0, PR, Off, Rs...>(lds, tw, io);
+}
+
+// run_fft_io over the tiles t0, t0 + dt, ... below t1 of one workgroup: the
+// next tile's first-stage loads are issued before the current tile's later
+// stages and its stores, so they are in flight while those run.  make_io(t)
+// gives tile t's addressing.
+template <class T, bool FWD, int B, int NT, bool PR, class Off, class MakeIO, int R0, int... Rs>
+__device__ __forceinline__ void run_fft_io_pipe(cpx<T>* lds, const typename TwT<T>::type* tw, int t0, int t1,
+                                                int dt, MakeIO make_io, RList<R0, Rs...>) {
+  static_assert(sizeof...(Rs) >= 1, "at least two stages");
+  constexpr int N = R0 * RProd<Rs...>::value;
+  if (t0 >= t1) return;
+  Stage0Regs<T, R0, N, B, NT> rg;
+  ColIO io = make_io(t0);
+  stage0_load<T, R0, N, B, NT, PR, Off>(rg, io);
+  for (int t = t0; t < t1; t += dt) {
+    stage0_put<T, R0, FWD, N, B, NT>(lds, rg);
+    const ColIO cur = io;
+    if (t + dt < t1) {
+      io = make_io(t + dt);
+      stage0_load<T, R0, N, B, NT, PR, Off>(rg, io);
+    }
+    mid_and_last<T, FWD, N, B, NT, R0, 0, PR, Off, Rs...>(lds, tw, cur);
+    // the last stage's LDS reads before the next tile's first-stage writes
+    __syncthreads();
+  }
 }
 
 // Forward FFT of a tile whose first stage is fed from global memory (with

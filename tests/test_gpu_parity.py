@@ -995,3 +995,35 @@ def test_stream_copy_exact(nvec):
     finally:
         a.free()
         b.free()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_col_pipe_z2048_against_oracle(precision):
+    """Z passes of length 2048 (126 x 36 x 2048: 288 FP32 / 576 FP64 tiles,
+    more than one per CU) through the persistent column kernel
+    (p3dfft_amd_set_col_pipe(1)), in the XCD-aware and the dispatch tile
+    order, against the oracle and equal to rounding to one workgroup per tile
+    (the same stage code, contracted to FMAs differently inside the loop)."""
+    nx, ny, nz = 126, 36, 2048
+    a = np.random.default_rng(2048).standard_normal((nz, ny, nx))
+    o = oplan(nx, ny, nz)
+    ref_f = o.ftran_r2c(a, "fft")
+    ref_b = o.btran_c2r(ref_f, "tff")[0]
+    lib = p3dfft_amd.load(precision)
+    res = {}
+    try:
+        for pipe, order in ((0, 2), (1, 0), (1, 1)):
+            lib.p3dfft_amd_set_col_pipe(pipe)
+            lib.p3dfft_amd_set_flat_order(order)
+            with lib_setup(precision, nx, ny, nz) as P:
+                f = P.ftran_r2c(a, "fft")
+                b = P.btran_c2r(ref_f, "tff")
+            assert rel(f, ref_f) < TOL[precision], (pipe, order)
+            assert rel(b, ref_b) < TOL[precision], (pipe, order)
+            res[(pipe, order)] = (f, b)
+        for key in ((1, 0), (1, 1)):
+            assert rel(res[key][0], res[(0, 2)][0]) < TOL[precision] / 10, key
+            assert rel(res[key][1], res[(0, 2)][1]) < TOL[precision] / 10, key
+    finally:
+        lib.p3dfft_amd_set_col_pipe(1)
+        lib.p3dfft_amd_set_flat_order(2)

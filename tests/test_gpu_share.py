@@ -317,6 +317,52 @@ def test_one_rank_of_2048cube_on_16x16_fp32_backward():
         S.clean()
 
 
+def test_col_pipe_2048cube_rank_fp32():
+    """The persistent column kernel (p3dfft_amd_set_col_pipe(1): one workgroup
+    per CU looping over tiles, the next tile's loads in flight while the
+    current one computes) against one workgroup per tile at rank (15, 3) of
+    2048^3 on 16x16: ftran_y / btran_y (table-addressed rows) and ftran_z /
+    btran_z (1040 tiles of length 2048) equal to rounding (the same stage
+    code, contracted to FMAs differently inside the loop), ftran_z against
+    numpy."""
+    n, dims, ip, jp = 2048, (16, 16), 15, 3
+    S = p3dfft_amd.Share(4)
+    S.setup_share(dims, ip, jp, n, n, n)
+    rng = np.random.default_rng(2050)
+
+    def crandn(m):
+        return (rng.standard_normal(m) + 1j * rng.standard_normal(m)).astype(np.complex64)
+    recv1, recv2, zp = crandn(S.n_ypen()), crandn(S.n_send2()), crandn(S.n_zpen())
+    outs = {}
+    try:
+        for pipe in (0, 1):
+            S.lib.p3dfft_amd_set_col_pipe(pipe)
+            r, s2 = dev(S, recv1), S.alloc(S.n_send2() * 8)
+            S.ftran_y(r, s2)
+            fy = get(S, s2, S.n_send2())
+            r.free(); s2.free()
+            r, sy = dev(S, recv2), S.alloc(S.n_ypen() * 8)
+            S.btran_y(r, sy)
+            by = get(S, sy, S.n_ypen())
+            r.free(); sy.free()
+            z = dev(S, zp)
+            S.ftran_z(z, z, "fft")
+            fz = get(S, z, S.n_zpen())
+            z.free()
+            z, snd = dev(S, zp), S.alloc(S.n_zpen() * 8)
+            S.btran_z(z, snd, "tff")
+            bz = get(S, snd, S.n_zpen())
+            z.free(); snd.free()
+            outs[pipe] = (fy, by, fz, bz)
+        for name, u, v in zip(("ftran_y", "btran_y", "ftran_z", "btran_z"), outs[0], outs[1]):
+            assert rel(u, v) < TOL[4] / 10, name
+        ref = np.fft.fft(zp.astype(np.complex128).reshape(n, 128, 65), axis=0)
+        assert rel(outs[1][2].reshape(n, 128, 65), ref) < TOL[4]
+    finally:
+        S.lib.p3dfft_amd_set_col_pipe(1)
+        S.clean()
+
+
 def coded_unique(n, seed):
     """n distinct complex values exact in FP32: the index split into two
     22-bit halves (a permutation error cannot hide between equal values)."""
