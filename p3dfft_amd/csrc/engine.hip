@@ -2349,12 +2349,14 @@ static bool col_fits32(const ColSpec& c, int TX, int rows) {
 static int g_col_pipe = 1;
 void set_col_pipe(int on) { g_col_pipe = on ? 1 : 0; }
 
-// workgroups of a persistent launch: one per CU of the current device
+// workgroups of a persistent launch: one per CU of the current device, and
+// at least min(ntiles, 8), so that each XCD's share of the tiles has a
+// workgroup (k_colio_pipe)
 static unsigned pipe_grid(long long ntiles) {
   int dev = 0, ncu = 0;
   check_hip(hipGetDevice(&dev), "hipGetDevice");
   check_hip(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
-  return (unsigned)std::max<long long>(1, std::min<long long>(ntiles, std::max(ncu, 1)));
+  return (unsigned)std::max<long long>(1, std::min<long long>(ntiles, std::max(ncu, 8)));
 }
 
 bool launch_col_tab(const ColSpec& c, const int2* ltab, const int2* stab, hipStream_t st) {
