@@ -189,12 +189,57 @@ __device__ __forceinline__ void stage0_put(cpx<T>* lds, Stage0Regs<T, R, N, B, N
 }
 
 // first stage: loads of one tile into registers, butterflies (no twiddles),
-// results into the LDS tile
+// results into the LDS tile (the loads as stage0_load, written out here so
+// that every load of the tile is issued back to back: composed of the two
+// steps, the compiler interleaved them with address arithmetic and waited
+// before the last one)
 template <class T, int R, bool FWD, int N, int B, int NT, bool PR, class Off>
 __device__ __forceinline__ void stage0_global(cpx<T>* lds, const ColIO& io) {
-  Stage0Regs<T, R, N, B, NT> rg;
-  stage0_load<T, R, N, B, NT, PR, Off>(rg, io);
-  stage0_put<T, R, FWD, N, B, NT>(lds, rg);
+  static_assert(NT % B == 0, "one column per thread");
+  constexpr bool TB = std::is_same<Off, TabRows>::value;
+  typedef Stage0<R, N, B, NT> S;
+  const bool valid = (int)(threadIdx.x % B) < io.nb;
+  cpx<T> v[S::IT][R];
+#pragma unroll
+  for (int m = 0; m < S::IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if ((S::total % NT) == 0 || it < S::total) {
+      const int j = it / B;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int e = j + r * S::nR;
+        // items outside the tile load its first element (always valid) and
+        // are zeroed, so no load sits under a branch
+        if constexpr (TB) {
+          const int2 t = io.ltab[e];
+          const bool ok = valid && t.x >= 0;
+          const unsigned el = ok ? (unsigned)t.x + io.lane_o * (unsigned)t.y + io.lane_i : 0u;
+          const cpx<T> u = ((const cpx<T>*)io.in)[el];
+          v[m][r] = ok ? u : mk<T>(0, 0);
+        } else if constexpr (PR) {
+          const int src = io.mirror ? (e < io.mirror ? e : 2 * (io.mirror - 1) - e)
+                                    : (e < io.ld_lo ? e : (e >= io.ld_hi ? e - io.ld_shift : -1));
+          const bool ok = valid && src >= 0;
+          const cpx<T> u = ld_at<T, Off>(io.in, ok ? (Off)io.lane_in + (Off)src * (Off)io.sl_in : (Off)0);
+          v[m][r] = ok ? u : mk<T>(0, 0);
+        } else {
+          const cpx<T> u = ld_at<T, Off>(io.in, valid ? (Off)io.lane_in + (Off)e * (Off)io.sl_in : (Off)0);
+          v[m][r] = valid ? u : mk<T>(0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < S::IT; m++) {
+    const int it = threadIdx.x + m * NT;
+    if ((S::total % NT) == 0 || it < S::total) {
+      const int j = it / B, b = it % B;
+      dft<R, FWD>(v[m]);
+#pragma unroll
+      for (int r = 0; r < R; r++) lds[(j * R + r) * B + b] = v[m][r];
+    }
+  }
+  __syncthreads();
 }
 
 // last stage: Ns * R == N, so every butterfly index j < Ns and k == j
