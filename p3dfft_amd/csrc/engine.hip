@@ -1077,8 +1077,10 @@ __global__ __launch_bounds__(NT, W) void k_colio_pipe(ColArgs a) {
 // top with that carry.  Columns fastest keep the lanes of one LDS access on
 // different banks; every LDS access of a thread is issued as a batch.
 constexpr int kChebyRM = 8;
+// Each thread stores its output rows straight to `out` (column b of the tile
+// at out + b*si_out, row e at + e*sl_out; columns b >= nb are not stored).
 template <class T, int TX, int NT>
-__device__ __forceinline__ void cheby_tile(cpx<T>* lds, double* wt, const ColArgs& a) {
+__device__ __forceinline__ void cheby_tile(cpx<T>* lds, double* wt, const ColArgs& a, cpx<T>* out, int nb) {
   static_assert(64 % TX == 0 && NT % 64 == 0, "whole column groups per wave");
   constexpr int P = NT / TX, CW = 64 / TX, NW = NT / 64, RM = kChebyRM;
   const int N = a.nst;
@@ -1134,18 +1136,18 @@ __device__ __forceinline__ void cheby_tile(cpx<T>* lds, double* wt, const ColArg
 #pragma unroll
       for (int k = 0; k < 4; k++) cr[k] += wt[(u * TX + b) * 4 + k];
   double2 ev = make_double2(cr[0], cr[1]), od = make_double2(cr[2], cr[3]);
+  cpx<T>* oc = out + (long long)b * a.si_out;
 #pragma unroll
   for (int i = RM - 1; i >= 0; i--) {
     const int r = lo + i;
     if (r < hi) {
       double2 d = (r & 1) ? ev : od;
       if (r == 0) { d.x *= 0.5; d.y *= 0.5; }
-      lds[addr[i]] = mk<T>((T)d.x, (T)d.y);
+      if (b < nb) oc[(long long)r * a.sl_out] = mk<T>((T)d.x, (T)d.y);
       if (r & 1) { od.x += x[i].x; od.y += x[i].y; }
       else { ev.x += x[i].x; ev.y += x[i].y; }
     }
   }
-  __syncthreads();
 }
 
 // Column transforms through an LDS load/store pass: DCT-I / DST-I
@@ -1212,7 +1214,11 @@ __global__ __launch_bounds__(NT, W) void k_col_ct(ColArgs a) {
   }
   if (a.kind != COL_COPY && !a.mirror) run_fft_ct<T, FWD, TX, TX, NT>(lds, tw, RL{});
   if constexpr (64 % TX == 0 && NT % 64 == 0) {
-    if (a.cheby) cheby_tile<T, TX, NT>(lds, (double*)(g_smem + a.cheby_off), a);
+    if (a.cheby) {
+      // the recurrence stores the output rows itself
+      cheby_tile<T, TX, NT>(lds, (double*)(g_smem + a.cheby_off), a, out, nb);
+      return;
+    }
   }
   for (int q = threadIdx.x; q < TX * a.nst; q += NT) {
     int b, e;
