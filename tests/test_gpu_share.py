@@ -229,6 +229,33 @@ def test_all_ranks_fused_stages(precision, dims):
     assert rel(field, field0) < TOL[precision] / 10
 
 
+@pytest.mark.parametrize("precision", [8, 4])
+def test_all_ranks_y2048_persistent_table_kernel(precision):
+    """A 2048-point Y axis on a 2 x 2 grid: every rank's fused Y stages run
+    the table-addressed persistent column kernel (fewer tiles than XCDs, the
+    last tile partial), in both precisions.  All ranks against the
+    single-rank oracle, and to rounding against one workgroup per tile."""
+    nx, ny, nz, dims = 10, 2048, 12, (2, 2)
+    a = np.random.default_rng(7).standard_normal((nz, ny, nx))
+    lib = p3dfft_amd.load(precision)
+    runs = []
+    for pipe in (1, 0):
+        lib.p3dfft_amd_set_col_pipe(pipe)
+        try:
+            runs.append(run_grid_on_gpu(precision, dims, nx, ny, nz, a))
+        finally:
+            lib.p3dfft_amd_set_col_pipe(1)
+    (spec, field, _), (spec0, field0, _) = runs
+    full = O.Plan(nx, ny, nz).ftran_r2c(a)
+    for (i, j), s in spec.items():
+        o = O.Plan(nx, ny, nz, dims=dims, ipid=i, jpid=j)
+        ref = full[:, o.jjstart - 1:o.jjend, o.iistart - 1:o.iiend].reshape(-1)
+        assert rel(s, ref) < TOL[precision], (i, j)
+        assert rel(s, spec0[(i, j)]) < TOL[precision] / 10, (i, j)
+    assert rel(field, a * nx * ny * nz) < TOL[precision] * 10
+    assert rel(field, field0) < TOL[precision] / 10
+
+
 def test_one_rank_of_2048cube_on_16x16_fp32():
     """Rank (15, 3) of 2048^3 on 16x16 (local X pencil (2048,128,128), the last
     row rank holds 65 kx): each stage's send buffer against numpy FFTs and the
