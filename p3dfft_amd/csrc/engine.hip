@@ -426,13 +426,7 @@ __device__ __forceinline__ void x_r2c_ct_tile(cpx<T>* lds, const typename TwT<T>
     const cpx<T> zk = lds[k * BP + b], zn = lds[(k == 0 ? 0 : N - k) * BP + b];
     const unsigned o = (unsigned)(2 * b) * so + (unsigned)k * (unsigned)sizeof(cpx<T>);
     const cpx<T> ya = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h), yb = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
-    if constexpr (XT) {
-      const int2 tk = a.xt[k];
-      const long long r = 2 * (pair0 + b);
-      cpx<T>* oc = (cpx<T>*)a.out;
-      if (2 * b < nr) oc[tk.x + r * tk.y] = ya;
-      if (2 * b + 1 < nr) oc[tk.x + (r + 1) * tk.y] = yb;
-    } else if (full) {
+    if (full) {
       *(cpx<T>*)(out + o) = ya;
       *(cpx<T>*)(out + o + so) = yb;
     } else {
@@ -441,7 +435,33 @@ __device__ __forceinline__ void x_r2c_ct_tile(cpx<T>* lds, const typename TwT<T>
     }
   };
   constexpr int H = N / 2;
-  if (!XT && (N % 2) == 0 && a.nxo == H + 1) {
+  if constexpr (XT) {
+    // the table entries of all this thread's modes first, then the stores:
+    // a store placed after its entry's load waits for every memory access
+    // issued before it, which otherwise made each item a round trip
+    constexpr int LS = (B * (H + 1) + NT - 1) / NT;  // nxo <= N/2 + 1
+    int2 tks[LS];
+#pragma unroll
+    for (int i = 0; i < LS; i++) {
+      const int q = threadIdx.x + i * NT;
+      const bool ok = q < B * a.nxo;
+      const int b = ok ? fdiv(q, a.dnxo) : 0;
+      tks[i] = a.xt[ok ? q - b * a.nxo : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < LS; i++) {
+      const int q = threadIdx.x + i * NT;
+      if (q < B * a.nxo) {
+        const int b = fdiv(q, a.dnxo), k = q - b * a.nxo;
+        const cpx<T> zk = lds[k * BP + b], zn = lds[(k == 0 ? 0 : N - k) * BP + b];
+        const cpx<T> ya = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h), yb = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
+        const long long r = 2 * (pair0 + b);
+        cpx<T>* oc = (cpx<T>*)a.out;
+        if (2 * b < nr) oc[tks[i].x + r * tks[i].y] = ya;
+        if (2 * b + 1 < nr) oc[tks[i].x + (r + 1) * tks[i].y] = yb;
+      }
+    }
+  } else if ((N % 2) == 0 && a.nxo == H + 1) {
     // every mode stored (no X cut): modes k < N/2 as item b*(N/2) + k
     // (compile-time divisor), the Nyquist mode of pair b by thread b
     for (int q = threadIdx.x; q < B * H; q += NT) put(q / H, q % H);
@@ -518,6 +538,19 @@ __device__ __forceinline__ void x_c2r_ct_tile(cpx<T>* lds, const typename TwT<T>
     constexpr int K = H + 1;
     constexpr int LI = (B * K + NT - 1) / NT;
     cpx<T> va[LI], vb[LI];
+    // table-addressed modes: every table entry of this thread first, then the
+    // data loads (a load placed after its entry's load waits for every access
+    // issued before it, which otherwise made each item a round trip)
+    int2 tks[XT ? LI : 1];
+    if constexpr (XT) {
+#pragma unroll
+      for (int i = 0; i < LI; i++) {
+        const int q = threadIdx.x + i * NT;
+        const int k = q % K;
+        const bool ok = (((B * K) % NT == 0) || q < B * K) && k < a.nxo;
+        tks[i] = a.xt[ok ? k : 0];
+      }
+    }
 #pragma unroll
     for (int i = 0; i < LI; i++) {
       const int q = threadIdx.x + i * NT;
@@ -528,7 +561,7 @@ __device__ __forceinline__ void x_c2r_ct_tile(cpx<T>* lds, const typename TwT<T>
       const bool oka = ok && 2 * b < nr, okb = ok && 2 * b + 1 < nr;
       if constexpr (XT) {
         const cpx<T>* ic = (const cpx<T>*)a.in;
-        const int2 tk = a.xt[ok ? k : 0];
+        const int2 tk = tks[i];
         const long long r = 2 * (pair0 + b);
         const cpx<T> ua = ic[oka ? tk.x + r * tk.y : 0];
         const cpx<T> ub = ic[okb ? tk.x + (r + 1) * tk.y : 0];
