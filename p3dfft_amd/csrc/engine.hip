@@ -421,18 +421,13 @@ __device__ __forceinline__ void x_r2c_ct_tile(cpx<T>* lds, const typename TwT<T>
   const unsigned so = (unsigned)(a.so_row * sizeof(cpx<T>));
   const T h = (T)0.5;
   const bool full = nr == 2 * B;  // uniform: no per-store row guard
-  // separation of mode k of pair b and its stores
-  auto put = [&](int b, int k) {
+  // separation of mode k of pair b and its stores (FULL: no row guard)
+  auto put = [&](int b, int k, auto fl) {
     const cpx<T> zk = lds[k * BP + b], zn = lds[(k == 0 ? 0 : N - k) * BP + b];
     const unsigned o = (unsigned)(2 * b) * so + (unsigned)k * (unsigned)sizeof(cpx<T>);
     const cpx<T> ya = mk<T>((zk.x + zn.x) * h, (zk.y - zn.y) * h), yb = mk<T>((zk.y + zn.y) * h, (zn.x - zk.x) * h);
-    if (full) {
-      *(cpx<T>*)(out + o) = ya;
-      *(cpx<T>*)(out + o + so) = yb;
-    } else {
-      if (2 * b < nr) *(cpx<T>*)(out + o) = ya;
-      if (2 * b + 1 < nr) *(cpx<T>*)(out + o + so) = yb;
-    }
+    if (decltype(fl)::value || 2 * b < nr) *(cpx<T>*)(out + o) = ya;
+    if (decltype(fl)::value || 2 * b + 1 < nr) *(cpx<T>*)(out + o + so) = yb;
   };
   constexpr int H = N / 2;
   if constexpr (XT) {
@@ -464,12 +459,23 @@ __device__ __forceinline__ void x_r2c_ct_tile(cpx<T>* lds, const typename TwT<T>
   } else if ((N % 2) == 0 && a.nxo == H + 1) {
     // every mode stored (no X cut): modes k < N/2 as item b*(N/2) + k
     // (compile-time divisor), the Nyquist mode of pair b by thread b
-    for (int q = threadIdx.x; q < B * H; q += NT) put(q / H, q % H);
-    if (threadIdx.x < B) put(threadIdx.x, H);
+    // a compile-time trip count and the full-tile test outside the loop, so
+    // that the LDS reads of all items can be issued ahead of their stores
+    constexpr int LS = (B * H + NT - 1) / NT;
+    auto put_all = [&](auto fl) {
+#pragma unroll
+      for (int i = 0; i < LS; i++) {
+        const int q = threadIdx.x + i * NT;
+        if ((B * H) % NT == 0 || q < B * H) put(q / H, q % H, fl);
+      }
+      if (threadIdx.x < B) put(threadIdx.x, H, fl);
+    };
+    if (full) put_all(std::true_type{});
+    else put_all(std::false_type{});
   } else {
     for (int q = threadIdx.x; q < B * a.nxo; q += NT) {
       const int b = fdiv(q, a.dnxo);
-      put(b, q - b * a.nxo);
+      put(b, q - b * a.nxo, std::false_type{});
     }
   }
 }
@@ -593,19 +599,24 @@ __device__ __forceinline__ void x_c2r_ct_tile(cpx<T>* lds, const typename TwT<T>
   run_fft_ct<T, false, B, BP, NT>(lds, tw, RL{});
   char* out = (char*)a.out + 2 * pout * a.so_row * (long long)sizeof(T);
   const unsigned so = (unsigned)(a.so_row * sizeof(T));
-  const bool full = nr == 2 * B;  // uniform: no per-store row guard
-  for (int q = threadIdx.x; q < B * N; q += NT) {
-    const int b = q / N, j = q % N;
-    const cpx<T> z = lds[j * BP + b];
-    const unsigned o = (unsigned)(2 * b) * so + (unsigned)j * (unsigned)sizeof(T);
-    if (full) {
-      *(T*)(out + o) = z.x;
-      *(T*)(out + o + so) = z.y;
-    } else {
-      if (2 * b < nr) *(T*)(out + o) = z.x;
-      if (2 * b + 1 < nr) *(T*)(out + o + so) = z.y;
+  // a compile-time trip count and the full-tile test outside the loop, so
+  // that the LDS reads of all items can be issued ahead of their stores
+  constexpr int LS = (B * N + NT - 1) / NT;
+  auto store_all = [&](auto full) {
+#pragma unroll
+    for (int i = 0; i < LS; i++) {
+      const int q = threadIdx.x + i * NT;
+      if ((B * N) % NT == 0 || q < B * N) {
+        const int b = q / N, j = q % N;
+        const cpx<T> z = lds[j * BP + b];
+        const unsigned o = (unsigned)(2 * b) * so + (unsigned)j * (unsigned)sizeof(T);
+        if (decltype(full)::value || 2 * b < nr) *(T*)(out + o) = z.x;
+        if (decltype(full)::value || 2 * b + 1 < nr) *(T*)(out + o + so) = z.y;
+      }
     }
-  }
+  };
+  if (nr == 2 * B) store_all(std::true_type{});  // uniform: no per-store row guard
+  else store_all(std::false_type{});
 }
 
 template <class T, int B, int NT, class RL, int W, bool TWL, bool XT>
