@@ -256,6 +256,57 @@ def test_all_ranks_y2048_persistent_table_kernel(precision):
     assert rel(field, field0) < TOL[precision] / 10
 
 
+@pytest.mark.parametrize("precision", [8, 4])
+@pytest.mark.parametrize("grid", [((2, 3), (32, 48, 16), (20, 30, 12)), ((3, 2), (64, 32, 48), (40, 20, 30))])
+def test_fused_stages_with_cuts_match_separate(precision, grid):
+    """Pruned grids (fewer stored x modes than nx/2+1, Y and Z cuts): on
+    every rank the six stages through the fused table-addressed kernels equal
+    to rounding the separate pack/unpack passes on the same random buffers;
+    those passes' maps are bit-exact against the oracle's
+    (test_pack_unpack_bit_exact), their FFTs the plain column and X kernels."""
+    dims, (nx, ny, nz), cut = grid
+    S = p3dfft_amd.Share(precision)
+    rng = np.random.default_rng(sum(grid[1]) + precision)
+
+    def crandn(m):
+        return (rng.standard_normal(m) + 1j * rng.standard_normal(m)).astype(S.cdtype)
+
+    def run(fn, inp, nout, real_in=False, real_out=False):
+        outs = []
+        for fused in (1, 0):
+            S.lib.p3dfft_share_set_fused(fused)
+            a = S.alloc(max(inp.size, 1) * inp.itemsize)
+            a.upload(inp)
+            isz = np.dtype(S.rdtype if real_out else S.cdtype).itemsize
+            b = S.alloc(max(nout, 1) * isz)
+            b.upload(np.zeros(nout, dtype=S.rdtype if real_out else S.cdtype))
+            fn(a, b)
+            outs.append(b.download(nout, S.rdtype if real_out else S.cdtype).astype(np.complex128))
+            a.free(); b.free()
+        return outs
+
+    try:
+        for ip in range(dims[0]):
+            for jp in range(dims[1]):
+                S.setup_share(dims, ip, jp, nx, ny, nz, *cut)
+                try:
+                    stages = [
+                        ("ftran_x", lambda a, b: S.ftran_x(a, b),
+                         rng.standard_normal(S.n_real()).astype(S.rdtype), S.n_send1(), False),
+                        ("ftran_y", lambda a, b: S.ftran_y(a, b), crandn(S.n_ypen()), S.n_send2(), False),
+                        ("btran_y", lambda a, b: S.btran_y(a, b), crandn(S.n_send2()), S.n_ypen(), False),
+                        ("btran_x", lambda a, b: S.btran_x(a, b), crandn(S.n_send1()), S.n_real(), True),
+                    ]
+                    for name, fn, inp, nout, real_out in stages:
+                        u, v = run(fn, inp, nout, real_out=real_out)
+                        assert np.linalg.norm(u) > 0, (name, ip, jp)
+                        assert rel(u, v) < TOL[precision] / 10, (name, ip, jp)
+                finally:
+                    S.clean()
+    finally:
+        S.lib.p3dfft_share_set_fused(1)
+
+
 def test_one_rank_of_2048cube_on_16x16_fp32():
     """Rank (15, 3) of 2048^3 on 16x16 (local X pencil (2048,128,128), the last
     row rank holds 65 kx): each stage's send buffer against numpy FFTs and the
