@@ -1393,7 +1393,7 @@ __device__ __forceinline__ int shell_of(int k2) {
 template <class T>
 __global__ __launch_bounds__(256) void k_spectrum(const cpx<T>* B, int d0, int d1, int f0, int f1, int f2,
                                                   int kmax, int nchunk, double* partial) {
-  constexpr int R = 4;  // consecutive elements per thread per step
+  constexpr int R = 8;  // consecutive elements per thread per step
   double* hist = (double*)g_smem;
   const int nw = blockDim.x / 64, w = threadIdx.x / 64, nb = kmax + 1;
   for (int i = threadIdx.x; i < nw * nb; i += blockDim.x) hist[i] = 0.0;
@@ -1409,29 +1409,40 @@ __global__ __launch_bounds__(256) void k_spectrum(const cpx<T>* B, int d0, int d
   const unsigned per = ((n + nchunk - 1) / nchunk + step - 1) / step * step;
   const unsigned lo = ch * per, hi = min(n, lo + per);
   for (unsigned i0 = lo + threadIdx.x * R; i0 < hi; i0 += step) {
+    // every load of the step first (items past the chunk load its last
+    // element and are skipped below), each element's shell and weighted
+    // energy computed outside any branch, so that the loads are issued back
+    // to back rather than one per branch with a wait after each
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = P[i0 + r < hi ? i0 + r : hi - 1];
     unsigned b = i0 / (unsigned)d0, a = i0 - b * (unsigned)d0;
     int kb = fold_k((int)b, f1);
+    int ik[R];
+    double e[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int ka = fold_k((int)a, f0);
+      const int k2 = ka * ka + kb * kb + kc2;
+      ik[r] = shell_of(k2);
+      e[r] = (double)k2 * ((double)v[r].x * (double)v[r].x + (double)v[r].y * (double)v[r].y);
+      if (++a == (unsigned)d0) {
+        a = 0;
+        b++;
+        kb = fold_k((int)b, f1);
+      }
+    }
     int cur = -1;
     double acc = 0.0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (i0 + r < hi) {
-        const cpx<T> v = P[i0 + r];
-        const int ka = fold_k((int)a, f0);
-        const int k2 = ka * ka + kb * kb + kc2;
-        const int ik = shell_of(k2);
-        const double e = (double)k2 * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
-        if (ik != cur) {
+        if (ik[r] != cur) {
           if (cur >= 0 && cur <= kmax) atomicAdd(hw + cur, acc);
-          cur = ik;
-          acc = e;
+          cur = ik[r];
+          acc = e[r];
         } else {
-          acc += e;
-        }
-        if (++a == (unsigned)d0) {
-          a = 0;
-          b++;
-          kb = fold_k((int)b, f1);
+          acc += e[r];
         }
       }
     }
