@@ -965,9 +965,24 @@ __global__ __launch_bounds__(NT) void k_xy_plane_fwd(PlaneArgs a) {
   const typename TwT<T>::type *tx, *ty;
   plane_twiddles<T, RLX, RLY, NT>(a, &tx, &ty);
   const T* in = (const T*)a.in + (long long)blockIdx.x * NX * NY;
-  for (int q = threadIdx.x; q < BX * NX; q += NT) {
-    const int b = q / NX, j = q % NX;
-    lx[j * BPX + b] = mk<T>(in[2 * b * NX + j], in[(2 * b + 1) * NX + j]);
+  {
+    // compile-time trip counts: every load of the plane issued before the
+    // LDS writes (items past the plane read element 0 and are not written)
+    constexpr int LI = (BX * NX + NT - 1) / NT;
+    T xa[LI], xb[LI];
+#pragma unroll
+    for (int i = 0; i < LI; i++) {
+      const int q = threadIdx.x + i * NT;
+      const bool ok = (BX * NX) % NT == 0 || q < BX * NX;
+      const int b = q / NX, j = q % NX;
+      xa[i] = in[ok ? 2 * b * NX + j : 0];
+      xb[i] = in[ok ? (2 * b + 1) * NX + j : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < LI; i++) {
+      const int q = threadIdx.x + i * NT;
+      if ((BX * NX) % NT == 0 || q < BX * NX) lx[(q % NX) * BPX + q / NX] = mk<T>(xa[i], xb[i]);
+    }
   }
   __syncthreads();
   run_fft_ct<T, true, BX, BPX, NT>(lx, tx, RLX{});
@@ -981,7 +996,18 @@ __global__ __launch_bounds__(NT) void k_xy_plane_fwd(PlaneArgs a) {
   __syncthreads();
   run_fft_ct<T, true, H, H, NT>(ly, ty, RLY{});
   cpx<T>* out = (cpx<T>*)a.out + (long long)blockIdx.x * H * NY;
-  for (int q = threadIdx.x; q < NY * H; q += NT) out[q] = ly[q];
+  constexpr int LQ = (NY * H + NT - 1) / NT;
+  cpx<T> v[LQ];
+#pragma unroll
+  for (int i = 0; i < LQ; i++) {
+    const int q = threadIdx.x + i * NT;
+    v[i] = ly[(NY * H) % NT == 0 || q < NY * H ? q : 0];
+  }
+#pragma unroll
+  for (int i = 0; i < LQ; i++) {
+    const int q = threadIdx.x + i * NT;
+    if ((NY * H) % NT == 0 || q < NY * H) out[q] = v[i];
+  }
 }
 
 // backward: Y backward and X c2r (as k_x_c2r_ct: Im of the DC and Nyquist
@@ -995,7 +1021,21 @@ __global__ __launch_bounds__(NT) void k_yx_plane_bwd(PlaneArgs a) {
   const typename TwT<T>::type *tx, *ty;
   plane_twiddles<T, RLX, RLY, NT>(a, &tx, &ty);
   const cpx<T>* in = (const cpx<T>*)a.in + (long long)blockIdx.x * H * NY;
-  for (int q = threadIdx.x; q < NY * H; q += NT) ly[q] = in[q];
+  {
+    // every load of the plane issued before the LDS writes
+    constexpr int LQ = (NY * H + NT - 1) / NT;
+    cpx<T> v[LQ];
+#pragma unroll
+    for (int i = 0; i < LQ; i++) {
+      const int q = threadIdx.x + i * NT;
+      v[i] = in[(NY * H) % NT == 0 || q < NY * H ? q : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < LQ; i++) {
+      const int q = threadIdx.x + i * NT;
+      if ((NY * H) % NT == 0 || q < NY * H) ly[q] = v[i];
+    }
+  }
   __syncthreads();
   run_fft_ct<T, false, H, H, NT>(ly, ty, RLY{});
   for (int q = threadIdx.x; q < BX * H; q += NT) {
@@ -1009,11 +1049,22 @@ __global__ __launch_bounds__(NT) void k_yx_plane_bwd(PlaneArgs a) {
   __syncthreads();
   run_fft_ct<T, false, BX, BPX, NT>(lx, tx, RLX{});
   T* out = (T*)a.out + (long long)blockIdx.x * NX * NY;
-  for (int q = threadIdx.x; q < BX * NX; q += NT) {
-    const int b = q / NX, j = q % NX;
-    const cpx<T> z = lx[j * BPX + b];
-    out[2 * b * NX + j] = z.x;
-    out[(2 * b + 1) * NX + j] = z.y;
+  constexpr int LI = (BX * NX + NT - 1) / NT;
+  cpx<T> z[LI];
+#pragma unroll
+  for (int i = 0; i < LI; i++) {
+    const int q = threadIdx.x + i * NT;
+    const bool ok = (BX * NX) % NT == 0 || q < BX * NX;
+    z[i] = lx[ok ? (q % NX) * BPX + q / NX : 0];
+  }
+#pragma unroll
+  for (int i = 0; i < LI; i++) {
+    const int q = threadIdx.x + i * NT;
+    if ((BX * NX) % NT == 0 || q < BX * NX) {
+      const int b = q / NX, j = q % NX;
+      out[2 * b * NX + j] = z[i].x;
+      out[(2 * b + 1) * NX + j] = z[i].y;
+    }
   }
 }
 
