@@ -66,15 +66,18 @@ struct LgCol {
 };
 
 // scratch row e of chunk column c <- the view's row e (load map, extension)
-template <class T>
+// I: the index type of the element and column counters and their divisions
+// (unsigned where every index of the launch is below 2^31, else 64-bit)
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_gather(LgCol a, cpx<T>* A) {
   const cpx<T>* in = (const cpx<T>*)a.in;
-  const long long total = (long long)a.m * a.ncc;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long e = q / a.ncc, c = q - e * a.ncc;
-    const long long col = a.c0 + c;
-    const long long o = col / a.n_inner, i = col - o * a.n_inner;
-    const cpx<T>* base = in + o * a.so + i * a.si;
+  const I total = (I)((long long)a.m * a.ncc), ncc = (I)a.ncc, nin = (I)a.n_inner, c0 = (I)a.c0;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const I e0 = q / ncc, c = q - e0 * ncc;
+    const I col = c0 + c;
+    const I o = col / nin, i = col - o * nin;
+    const long long e = (long long)e0;
+    const cpx<T>* base = in + (long long)o * a.so + (long long)i * a.si;
     auto load = [&](long long r) -> cpx<T> {
       if (r < 0 || r >= a.nld) return mk<T>(0, 0);
       const long long src = r < a.ld_lo ? r : (r >= a.ld_hi ? r - a.ld_shift : -1);
@@ -98,30 +101,31 @@ __global__ __launch_bounds__(256) void k_lg_gather(LgCol a, cpx<T>* A) {
 
 // the view's output row e <- scratch row p(e) (store map; DST-I output is
 // -i times row p+1 of the odd extension's transform)
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_scatter(LgCol a, const cpx<T>* R) {
   cpx<T>* out = (cpx<T>*)a.out;
-  const long long total = (long long)a.nst * a.ncc;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long e = q / a.ncc, c = q - e * a.ncc;
-    const long long col = a.c0 + c;
-    const long long o = col / a.n_inner, i = col - o * a.n_inner;
+  const I total = (I)((long long)a.nst * a.ncc), ncc = (I)a.ncc, nin = (I)a.n_inner, c0 = (I)a.c0;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const I e0 = q / ncc, c = q - e0 * ncc;
+    const I col = c0 + c;
+    const I o = col / nin, i = col - o * nin;
+    const long long e = (long long)e0;
     const long long p = e < a.st_lo ? e : e + a.st_shift;
     cpx<T> v;
     if (a.kind == COL_DST1) { const cpx<T> u = R[(p + 1) * a.ncc + c]; v = mk<T>(-u.y, u.x); }
     else v = R[p * a.ncc + c];
-    out[o * a.so + i * a.si + e * a.sl] = v;
+    out[(long long)o * a.so + (long long)i * a.si + e * a.sl] = v;
   }
 }
 
 // four-step twiddle: row n1 + N1*k2 of every column times W_m^(sgn n1 k2)
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_twiddle(cpx<T>* A, long long ncc, int N1, int m, int sgn) {
-  const long long total = (long long)m * ncc;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long row = q / ncc;
-    const long long k2 = row / N1, n1 = row - k2 * N1;
-    const long long r = n1 * k2;  // < m
+  const I total = (I)((long long)m * ncc), nc = (I)ncc;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const unsigned row = (unsigned)(q / nc);  // < m
+    const unsigned k2 = row / (unsigned)N1, n1 = row - k2 * (unsigned)N1;
+    const long long r = (long long)n1 * k2;  // < m
     if (r == 0) continue;
     const cpx<double> w = expipi(sgn < 0 ? -2 * r : 2 * r, m);
     A[q] = cmul(A[q], to_t<T>(w));
@@ -129,21 +133,21 @@ __global__ __launch_bounds__(256) void k_lg_twiddle(cpx<T>* A, long long ncc, in
 }
 
 // Bluestein: P[j] = x_j c_j (j < m), 0 up to M
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_blu_pre(const cpx<T>* A, cpx<T>* P, long long ncc, int m, int M, int sgn) {
-  const long long total = (long long)M * ncc;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long j = q / ncc;
+  const I total = (I)((long long)M * ncc), nc = (I)ncc;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const long long j = (long long)(q / nc);
     P[q] = j < m ? cmul(A[q], to_t<T>(chirp(j, m, sgn))) : mk<T>(0, 0);
   }
 }
 
 // Bluestein: Q[j] *= H[j] (forward) or conj(H[(M-j) mod M]) (backward)
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_blu_mul(cpx<T>* Q, const cpx<T>* H, long long ncc, int M, int flip) {
-  const long long total = (long long)M * ncc;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long j = q / ncc;
+  const I total = (I)((long long)M * ncc), nc = (I)ncc;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const long long j = (long long)(q / nc);
     cpx<T> h;
     if (flip) { const cpx<T> g = H[j == 0 ? 0 : M - j]; h = mk<T>(g.x, -g.y); }
     else h = H[j];
@@ -152,12 +156,12 @@ __global__ __launch_bounds__(256) void k_lg_blu_mul(cpx<T>* Q, const cpx<T>* H, 
 }
 
 // Bluestein: X_k = c_k Y_k / M (k < m)
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_blu_post(const cpx<T>* P, cpx<T>* A, long long ncc, int m, int M, int sgn) {
-  const long long total = (long long)m * ncc;
+  const I total = (I)((long long)m * ncc), nc = (I)ncc;
   const double inv = 1.0 / (double)M;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long k = q / ncc;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const long long k = (long long)(q / nc);
     const cpx<double> c = chirp(k, m, sgn);
     A[q] = cmul(P[q], to_t<T>(mk<double>(c.x * inv, c.y * inv)));
   }
@@ -176,12 +180,12 @@ __global__ __launch_bounds__(256) void k_lg_blu_filter(cpx<T>* h, int m, int M) 
 }
 
 // X axis, forward: scratch row e of pair p = (row 2p, row 2p+1) element e
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_x_gather(const T* in, cpx<T>* A, long long nrows, long long si_row,
                                                      int nx, long long p0, long long npc) {
-  const long long total = (long long)nx * npc;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long e = q / npc, p = q - e * npc;
+  const I total = (I)((long long)nx * npc), np = (I)npc;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const long long e = (long long)(q / np), p = (long long)(q - (q / np) * np);
     const long long r0 = 2 * (p0 + p);
     const T va = r0 < nrows ? in[r0 * si_row + e] : (T)0;
     const T vb = r0 + 1 < nrows ? in[(r0 + 1) * si_row + e] : (T)0;
@@ -190,12 +194,12 @@ __global__ __launch_bounds__(256) void k_lg_x_gather(const T* in, cpx<T>* A, lon
 }
 
 // X axis, forward: the two half spectra of each pair, modes k < nxo
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_x_separate(const cpx<T>* R, cpx<T>* out, long long nrows, long long so_row,
                                                        int nx, int nxo, long long p0, long long npc) {
-  const long long total = (long long)nxo * npc;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long k = q / npc, p = q - k * npc;
+  const I total = (I)((long long)nxo * npc), np = (I)npc;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const long long k = (long long)(q / np), p = (long long)(q - (q / np) * np);
     const cpx<T> z = R[q];
     const cpx<T> zm = R[(k == 0 ? 0 : nx - k) * npc + p];
     const T h = (T)0.5;
@@ -208,13 +212,13 @@ __global__ __launch_bounds__(256) void k_lg_x_separate(const cpx<T>* R, cpx<T>* 
 // X axis, backward: the full spectrum of (row 2p) + i (row 2p+1) from the
 // stored modes k < nxi (Hermitian fill above nx/2; Im of the DC and Nyquist
 // modes ignored; modes at and above nxi zero)
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_x_merge(const cpx<T>* in, cpx<T>* A, long long nrows, long long si_row,
                                                     int nx, int nxi, long long p0, long long npc) {
-  const long long total = (long long)nx * npc;
+  const I total = (I)((long long)nx * npc), np = (I)npc;
   const int half = nx / 2;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long k = q / npc, p = q - k * npc;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const long long k = (long long)(q / np), p = (long long)(q - (q / np) * np);
     const bool up = k > half;
     const long long kk = up ? nx - k : k;
     const bool real_mode = kk == 0 || (nx % 2 == 0 && kk == half);
@@ -232,18 +236,22 @@ __global__ __launch_bounds__(256) void k_lg_x_merge(const cpx<T>* in, cpx<T>* A,
 }
 
 // X axis, backward: real rows 2p, 2p+1 <- Re, Im of the inverse transform
-template <class T>
+template <class T, class I>
 __global__ __launch_bounds__(256) void k_lg_x_store(const cpx<T>* R, T* out, long long nrows, long long so_row, int nx,
                                                     long long p0, long long npc) {
-  const long long total = (long long)nx * npc;
-  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < total; q += (long long)gridDim.x * 256) {
-    const long long e = q / npc, p = q - e * npc;
+  const I total = (I)((long long)nx * npc), np = (I)npc;
+  for (I q = (I)blockIdx.x * 256 + threadIdx.x; q < total; q += (I)gridDim.x * 256) {
+    const long long e = (long long)(q / np), p = (long long)(q - (q / np) * np);
     const long long r0 = 2 * (p0 + p);
     const cpx<T> v = R[q];
     if (r0 < nrows) out[r0 * so_row + e] = v.x;
     if (r0 + 1 < nrows) out[(r0 + 1) * so_row + e] = v.y;
   }
 }
+
+// 32-bit index arithmetic where every index of a launch stays below 2^31
+// (the grid stride then cannot wrap either)
+bool idx32(long long n) { return n < (1LL << 31); }
 
 unsigned grid_for(long long total) {
   const long long g = (total + 255) / 256;
@@ -354,7 +362,10 @@ cplx_t* fft_flat(cplx_t* A, cplx_t* B, cplx_t* P, cplx_t* Q, long long ncc, int 
     const int N1 = L.N1, N2 = L.N2;
     // N2-point FFTs of the subsequences n1 (outer, stride ncc) over n2 (stride N1*ncc)
     flat_pass(A, A, ncc, (long long)N1 * ncc, ncc, (long long)N1 * ncc, ncc, N1, N2, fwd, st);
-    k_lg_twiddle<real_t><<<grid_for((long long)m * ncc), 256, 0, st>>>(A, ncc, N1, m, sgn);
+    if (idx32((long long)m * ncc))
+      k_lg_twiddle<real_t, unsigned><<<grid_for((long long)m * ncc), 256, 0, st>>>(A, ncc, N1, m, sgn);
+    else
+      k_lg_twiddle<real_t, unsigned long long><<<grid_for((long long)m * ncc), 256, 0, st>>>(A, ncc, N1, m, sgn);
     check_hip(hipGetLastError(), "k_lg_twiddle launch");
     // N1-point FFTs over n1 (stride ncc) of each k2 (outer, stride N1*ncc),
     // stored at row k2 + N2*k1
@@ -362,13 +373,17 @@ cplx_t* fft_flat(cplx_t* A, cplx_t* B, cplx_t* P, cplx_t* Q, long long ncc, int 
     return B;
   }
   const int M = L.M;
-  k_lg_blu_pre<real_t><<<grid_for((long long)M * ncc), 256, 0, st>>>(A, P, ncc, m, M, sgn);
+  const bool i32 = idx32((long long)M * ncc);  // M > m
+  if (i32) k_lg_blu_pre<real_t, unsigned><<<grid_for((long long)M * ncc), 256, 0, st>>>(A, P, ncc, m, M, sgn);
+  else k_lg_blu_pre<real_t, unsigned long long><<<grid_for((long long)M * ncc), 256, 0, st>>>(A, P, ncc, m, M, sgn);
   check_hip(hipGetLastError(), "k_lg_blu_pre launch");
   cplx_t* R = fft_flat(P, Q, nullptr, nullptr, ncc, M, true, st);
-  k_lg_blu_mul<real_t><<<grid_for((long long)M * ncc), 256, 0, st>>>(R, L.H, ncc, M, fwd ? 0 : 1);
+  if (i32) k_lg_blu_mul<real_t, unsigned><<<grid_for((long long)M * ncc), 256, 0, st>>>(R, L.H, ncc, M, fwd ? 0 : 1);
+  else k_lg_blu_mul<real_t, unsigned long long><<<grid_for((long long)M * ncc), 256, 0, st>>>(R, L.H, ncc, M, fwd ? 0 : 1);
   check_hip(hipGetLastError(), "k_lg_blu_mul launch");
   cplx_t* R2 = fft_flat(R, R == P ? Q : P, nullptr, nullptr, ncc, M, false, st);
-  k_lg_blu_post<real_t><<<grid_for((long long)m * ncc), 256, 0, st>>>(R2, A, ncc, m, M, sgn);
+  if (i32) k_lg_blu_post<real_t, unsigned><<<grid_for((long long)m * ncc), 256, 0, st>>>(R2, A, ncc, m, M, sgn);
+  else k_lg_blu_post<real_t, unsigned long long><<<grid_for((long long)m * ncc), 256, 0, st>>>(R2, A, ncc, m, M, sgn);
   check_hip(hipGetLastError(), "k_lg_blu_post launch");
   return A;
 }
@@ -412,7 +427,7 @@ LongPlan build_long(int m) {
   // one column, four-step (M is 5-smooth and splits)
   const int sgn = -1;
   flat_pass(h, h, 1, a, 1, a, 1, a, b, true, 0);
-  k_lg_twiddle<real_t><<<grid_for(M), 256, 0, 0>>>(h, 1, a, (int)M, sgn);
+  k_lg_twiddle<real_t, unsigned><<<grid_for(M), 256, 0, 0>>>(h, 1, a, (int)M, sgn);  // M < 2^27
   check_hip(hipGetLastError(), "k_lg_twiddle launch");
   flat_pass(h, t, a, 1, 1, b, 1, b, a, true, 0);
   check_hip(hipDeviceSynchronize(), "Bluestein filter");
@@ -473,11 +488,15 @@ void launch_col_long(const ColSpec& c, int m, hipStream_t st) {
     g.c0 = c0;
     g.ncc = std::min(ncc, ncol - c0);
     g.m = m;
-    k_lg_gather<real_t><<<grid_for((long long)m * g.ncc), 256, 0, st>>>(g, A);
+    // 32-bit counters: element index (rows x chunk columns) and flattened column
+    const bool i32 = idx32((long long)std::max(m, c.nst) * g.ncc) && idx32(ncol);
+    if (i32) k_lg_gather<real_t, unsigned><<<grid_for((long long)m * g.ncc), 256, 0, st>>>(g, A);
+    else k_lg_gather<real_t, unsigned long long><<<grid_for((long long)m * g.ncc), 256, 0, st>>>(g, A);
     check_hip(hipGetLastError(), "k_lg_gather launch");
     const cplx_t* R = fft_flat(A, B, P, Q, g.ncc, m, fwd, st);
     g.si = c.si_out; g.sl = c.sl_out; g.so = c.so_out;
-    k_lg_scatter<real_t><<<grid_for((long long)c.nst * g.ncc), 256, 0, st>>>(g, R);
+    if (i32) k_lg_scatter<real_t, unsigned><<<grid_for((long long)c.nst * g.ncc), 256, 0, st>>>(g, R);
+    else k_lg_scatter<real_t, unsigned long long><<<grid_for((long long)c.nst * g.ncc), 256, 0, st>>>(g, R);
     check_hip(hipGetLastError(), "k_lg_scatter launch");
   }
 }
@@ -496,18 +515,26 @@ void launch_x_long(bool r2c, const void* in, void* out, long long nrows, long lo
   for (long long p0 = 0; p0 < pairs; p0 += npc) {
     const long long n = std::min(npc, pairs - p0);
     const unsigned gx = grid_for((long long)nx * n);
+    const bool i32 = idx32((long long)nx * n);  // nk <= nx
     if (r2c) {
-      k_lg_x_gather<real_t><<<gx, 256, 0, st>>>((const real_t*)in, A, nrows, si_row, nx, p0, n);
+      if (i32) k_lg_x_gather<real_t, unsigned><<<gx, 256, 0, st>>>((const real_t*)in, A, nrows, si_row, nx, p0, n);
+      else k_lg_x_gather<real_t, unsigned long long><<<gx, 256, 0, st>>>((const real_t*)in, A, nrows, si_row, nx, p0, n);
       check_hip(hipGetLastError(), "k_lg_x_gather launch");
       const cplx_t* R = fft_flat(A, B, P, Q, n, nx, true, st);
-      k_lg_x_separate<real_t><<<grid_for((long long)nk * n), 256, 0, st>>>(R, (cplx_t*)out, nrows, so_row, nx, nk,
-                                                                             p0, n);
+      if (i32)
+        k_lg_x_separate<real_t, unsigned><<<grid_for((long long)nk * n), 256, 0, st>>>(R, (cplx_t*)out, nrows, so_row,
+                                                                                       nx, nk, p0, n);
+      else
+        k_lg_x_separate<real_t, unsigned long long><<<grid_for((long long)nk * n), 256, 0, st>>>(
+            R, (cplx_t*)out, nrows, so_row, nx, nk, p0, n);
       check_hip(hipGetLastError(), "k_lg_x_separate launch");
     } else {
-      k_lg_x_merge<real_t><<<gx, 256, 0, st>>>((const cplx_t*)in, A, nrows, si_row, nx, nk, p0, n);
+      if (i32) k_lg_x_merge<real_t, unsigned><<<gx, 256, 0, st>>>((const cplx_t*)in, A, nrows, si_row, nx, nk, p0, n);
+      else k_lg_x_merge<real_t, unsigned long long><<<gx, 256, 0, st>>>((const cplx_t*)in, A, nrows, si_row, nx, nk, p0, n);
       check_hip(hipGetLastError(), "k_lg_x_merge launch");
       const cplx_t* R = fft_flat(A, B, P, Q, n, nx, false, st);
-      k_lg_x_store<real_t><<<gx, 256, 0, st>>>(R, (real_t*)out, nrows, so_row, nx, p0, n);
+      if (i32) k_lg_x_store<real_t, unsigned><<<gx, 256, 0, st>>>(R, (real_t*)out, nrows, so_row, nx, p0, n);
+      else k_lg_x_store<real_t, unsigned long long><<<gx, 256, 0, st>>>(R, (real_t*)out, nrows, so_row, nx, p0, n);
       check_hip(hipGetLastError(), "k_lg_x_store launch");
     }
   }
