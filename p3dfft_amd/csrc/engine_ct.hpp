@@ -264,13 +264,24 @@ __device__ __forceinline__ void stageL_global(const cpx<T>* lds, const typename 
 #pragma unroll
       for (int r = 1; r < R; r++) v[r] = twmul<FWD>(v[r], w[r - 1]);
       dft<R, FWD>(v);
+      // table-addressed rows: every entry and address first, so the table
+      // reads issue together instead of one read and wait per store
+      unsigned el[TB ? R : 1];
+      bool st[TB ? R : 1];
+      if constexpr (TB) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int2 t = io.stab[j + r * Ns];
+          el[r] = (unsigned)t.x + io.lane_o * (unsigned)t.y + io.lane_i;
+          st[r] = t.x >= 0;
+        }
+      }
       if (valid) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
           const int p = j + r * Ns;
           if constexpr (TB) {
-            const int2 t = io.stab[p];
-            if (t.x >= 0) ((cpx<T>*)io.out)[(unsigned)t.x + io.lane_o * (unsigned)t.y + io.lane_i] = v[r];
+            if (st[r]) ((cpx<T>*)io.out)[el[r]] = v[r];
           } else if constexpr (PR) {
             const int e = p < io.st_lo ? p : p - io.st_shift;
             if (p < io.st_lo || (p >= io.st_lo + io.st_shift && e < io.nst))
